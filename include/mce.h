@@ -1,0 +1,216 @@
+/*
+ * mce.h — C ABI of the MI355X maximal clique enumeration engine (libmce.so).
+ *
+ * Plain C: fixed-width integers, plain pointers and sizes, no exceptions and
+ * no exit() across the boundary. Every entry point returns 0 on success or a
+ * negative MCE_E* code; mce_strerror() turns a code into text and
+ * mce_last_error() gives the detail of the last failure on the calling thread.
+ *
+ * Each declaration cites the reference interface it replaces
+ * (yen-hsiang-chang/mce-gpu, path:line).
+ *
+ * Semantics shared by every counting entry point:
+ *   - the graph is simple and undirected, given as a symmetric CSR whose rows
+ *     are sorted ascending with no duplicates and no self loops;
+ *   - maximal cliques with at least 2 vertices are counted (isolated
+ *     vertices are not cliques here; reference mce/mce_kernel_nowl.cuh:61-69);
+ *   - hist[k] is the number of maximal cliques with k vertices;
+ *   - hash = sum over cliques C of fmix64(sum_{v in C} splitmix64(v) +
+ *     |C| * 0x9E3779B97F4A7C15) mod 2^64, where v are the CSR vertex ids.
+ *     It is a symmetric function of the clique, so it does not depend on the
+ *     enumeration order, the variant or the schedule.
+ */
+#ifndef MCE_H
+#define MCE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCE_ABI_VERSION 1
+
+/* ---- error codes ------------------------------------------------------ */
+#define MCE_OK 0
+#define MCE_EINVAL -1    /* bad argument or option combination            */
+#define MCE_EIO -2       /* file missing, unreadable or truncated           */
+#define MCE_ENOMEM -3    /* host or device allocation failed                */
+#define MCE_EDEVICE -4   /* HIP runtime error or no usable GPU              */
+#define MCE_ELIMIT -5    /* a size limit of the engine was exceeded         */
+#define MCE_EGRAPH -6    /* malformed CSR (unsorted row, self loop, ...)    */
+#define MCE_EMODEL -7    /* cost model file missing or malformed            */
+#define MCE_EINTERNAL -8 /* an internal invariant failed (a bug)            */
+
+/* ---- enums mirroring the reference command line (include/defs.h:12-40) */
+enum mce_level { MCE_L1 = 1, MCE_L2 = 2 };       /* -p l1 | l2  */
+enum mce_induced { MCE_IP = 1, MCE_IPX = 2 };    /* -i p | px   */
+enum mce_worklist { MCE_NOWL = 1, MCE_WL = 2 };  /* -w nowl | wl */
+enum mce_task {                                  /* -m ...       */
+  MCE_TASK_MCE = 0,        /* mce                                  */
+  MCE_TASK_LB_EVAL = 1,    /* mce-lb-eval: tree nodes per wave       */
+  MCE_TASK_BD_EVAL = 2,    /* mce-bd-eval: cycle breakdown per wave  */
+  MCE_TASK_DONOR_EVAL = 3  /* mce-donor-eval: donations per wave     */
+};
+enum mce_schedule {
+  MCE_SCHED_PLAIN = 0,   /* heaviest-first by out-degree (the fallback)      */
+  MCE_SCHED_LEARNED = 1, /* heaviest-first by the MLP cost model             */
+  MCE_SCHED_GIVEN = 2    /* caller-given order of first-level subproblems    */
+};
+
+/* Largest out-degree (bitmap width) the engine accepts: 64 words of 64 bits. */
+#define MCE_MAX_OUT_DEGREE 4096u
+
+/* ---- graph in host memory --------------------------------------------- */
+/* CSR. row_ptr has n+1 entries, col_ind has m = row_ptr[n] entries (arcs).
+ * Replaces graph::CSRCOO (include/csrcoo.cuh:16-107); offsets are 64-bit so
+ * m may exceed 2^32 (the reference holds m in uint32, fileop.h:96). */
+typedef struct mce_csr {
+  uint32_t n;
+  uint64_t m;
+  uint64_t *row_ptr;
+  uint32_t *col_ind;
+} mce_csr;
+
+/* A batch of independent graphs packed into one CSR: graph g owns vertices
+ * [vtx_off[g], vtx_off[g+1]) and its arcs only join vertices of graph g. */
+typedef struct mce_batch {
+  uint32_t num_graphs;
+  uint32_t *vtx_off; /* num_graphs + 1 entries */
+  mce_csr csr;
+} mce_batch;
+
+/* ---- run options ------------------------------------------------------ */
+typedef struct mce_opts {
+  int level;    /* enum mce_level      */
+  int induced;  /* enum mce_induced    */
+  int worklist; /* enum mce_worklist   */
+  int task;     /* enum mce_task       */
+  int device;   /* HIP device ordinal; the engine uses exactly one GPU */
+
+  /* block settings (0 = automatic) */
+  int waves_per_group;     /* wave64s per workgroup (1..16)                 */
+  int groups_per_cu;       /* resident workgroups per CU                    */
+  int lds_arena_bytes;     /* LDS bytes per wave: bitmap cutoff for LDS     */
+  int hbm_arena_bytes;     /* HBM spill arena per wave for larger problems  */
+  int stack_lds_levels;    /* BK frames kept in LDS before spilling to HBM  */
+  int donate_min_p;        /* donate a child only if |P| >= this (SZTH)     */
+  int donate_check_nodes;  /* tree nodes between polls of the idle count    */
+
+  int schedule;            /* enum mce_schedule                              */
+  const uint32_t *order;   /* MCE_SCHED_GIVEN: first-level vertex order      */
+  uint64_t order_len;
+  const char *model_path;  /* MCE_SCHED_LEARNED: cost model file             */
+
+  /* optional outputs (NULL = not collected) */
+  uint64_t *sub_nodes;     /* n entries: BK tree nodes per first-level vertex */
+  uint64_t *sub_cliques;   /* n entries: maximal cliques per first-level vertex */
+} mce_opts;
+
+/* Per-wave record of the evaluation tasks (lb / donor / bd). */
+typedef struct mce_wave_stat {
+  uint32_t wave;   /* global wave slot                                   */
+  uint32_t cu;     /* compute unit id (XCC_ID * 256 + HW_ID CU fields)    */
+  uint64_t value;  /* tree nodes (lb), donations (donor) or cycles (bd)  */
+  uint64_t bd[7];  /* bd only: other, induced, worker list, pivots,
+                      set ops, maximality, branching (wl_bd.cuh:8-18)    */
+} mce_wave_stat;
+
+typedef struct mce_result {
+  uint64_t count;            /* maximal cliques (size >= 2)              */
+  uint64_t hash;             /* order-independent clique hash            */
+  uint64_t *hist;            /* caller-owned, hist_len entries, or NULL  */
+  uint32_t hist_len;
+  uint32_t max_clique_size;
+  uint32_t degeneracy;       /* max core number ("Max Core", kcore.cuh:315) */
+  uint32_t max_degree;       /* Delta                                   */
+  uint32_t max_out_degree;   /* Delta+ after degeneracy orientation     */
+  uint32_t num_waves;        /* resident waves of the enumeration launch */
+  uint64_t tree_nodes;       /* BK tree nodes expanded                  */
+  uint64_t donations;        /* worker-list donations                   */
+  uint64_t fallback_events;  /* learned schedule fell back to plain     */
+  double t_order_ms;         /* k-core ordering, device time            */
+  double t_orient_ms;        /* orientation + degree maxima, device time */
+  double t_count_ms;         /* enumeration, device time                */
+  mce_wave_stat *wave_stats; /* caller-owned, wave_stats_cap entries    */
+  uint32_t wave_stats_cap;
+  uint32_t wave_stats_len;
+} mce_result;
+
+/* ---- library ---------------------------------------------------------- */
+int mce_abi_version(void);
+const char *mce_strerror(int code);
+const char *mce_last_error(void);
+void mce_opts_default(mce_opts *opts);
+/* Number of HIP devices (0 when none). Replaces CUDAContext (utils.cuh:37-60). */
+int mce_device_count(void);
+
+/* ---- host graph construction (no GPU needed) -------------------------- */
+/* Reads a .bel file: 24-byte records {dst u64, src u64, 0 u64}, sorted by
+ * (src, dst); n = max src + 1. Replaces graph::read_bel (fileop.h:81-129)
+ * followed by CSRCOO::from_edgelist (csrcoo.cuh:41-81). */
+int mce_csr_read_bel(const char *path, mce_csr *out);
+/* Writes the .bel layout above (fileop.h:66-68). */
+int mce_csr_write_bel(const char *path, const mce_csr *csr);
+/* Text edge list -> .bel: symmetrize, drop self loops, sort, dedup.
+ * Replaces graph::convert_to_bel (fileop.h:13-79); prints nothing. */
+int mce_convert_text_to_bel(const char *text_path, const char *bel_path,
+                            uint64_t *n_out, uint64_t *m_undirected_out);
+/* Builds a simple symmetric CSR from an arbitrary edge list (both
+ * directions added, self loops and duplicates dropped). n may be 0 to use
+ * max id + 1. */
+int mce_csr_from_edges(uint32_t n, const uint32_t *src, const uint32_t *dst,
+                       uint64_t num_edges, mce_csr *out);
+/* Checks the CSR contract (sorted rows, symmetric, no loops, no dups). */
+int mce_csr_validate(const mce_csr *csr);
+void mce_csr_free(mce_csr *csr);
+void mce_batch_free(mce_batch *batch);
+
+/* Seeded synthetic generators (deterministic for a given seed). */
+int mce_gen_er(uint32_t n, uint64_t m_edges, uint64_t seed, mce_csr *out);
+int mce_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b,
+                 double c, uint64_t seed, mce_csr *out);
+int mce_gen_chung_lu(uint32_t n, double beta, double mean_degree,
+                     uint32_t planted, uint32_t k_min, uint32_t k_max,
+                     uint64_t seed, mce_csr *out);
+int mce_gen_sbm_batch(uint32_t num_graphs, uint32_t n_min, uint32_t n_max,
+                      uint32_t blocks, double p_in, double p_out,
+                      uint64_t seed, mce_batch *out);
+/* 64-bit digest of a CSR (n, m, row_ptr, col_ind) for fixtures. */
+uint64_t mce_csr_digest(const mce_csr *csr);
+
+/* ---- device pipeline -------------------------------------------------- */
+typedef struct mce_graph mce_graph; /* device-resident graph, one GPU */
+
+/* Uploads the CSR to HBM on opts->device. Replaces the managed-memory upload
+ * of src/main.cu:44-95. */
+int mce_graph_create(const mce_csr *csr, int device, mce_graph **out);
+/* Degeneracy ordering by k-core peeling on the GPU, then orientation: each
+ * row becomes [earlier neighbours | later neighbours], both ascending, where
+ * w is later than u iff (pri[u], u) < (pri[w], w). Replaces
+ * SingleGPU_Kcore::findKcoreIncremental_async (kcore.cuh:233-316) and the
+ * set_priority/split_pointer/split_data passes (main_support.cuh:6-84,
+ * src/main.cu:102-146). Fills degeneracy, max degrees and t_order/t_orient. */
+int mce_graph_order(mce_graph *g, mce_result *res);
+/* Enumerates maximal cliques of an ordered graph. Replaces
+ * MultiGPU_MCE::mce_count (mce/mce.cuh:107-415) + sync + show (:417-422). */
+int mce_graph_count(mce_graph *g, const mce_opts *opts, mce_result *res);
+/* Copies the ordering back: priority[n], split[n] (start of the later
+ * neighbours in each row) and the oriented col_ind[m]. Any may be NULL. */
+int mce_graph_download(const mce_graph *g, uint32_t *priority,
+                       uint64_t *split, uint32_t *col_ind);
+void mce_graph_destroy(mce_graph *g);
+
+/* One call: create + order + count + destroy (src/main.cu:18-187). */
+int mce_run(const mce_csr *csr, const mce_opts *opts, mce_result *res);
+
+/* Batched engine: all graphs of a packed batch in one persistent,
+ * graph-captured launch; counts_out[g] receives graph g's clique count. */
+int mce_run_batch(const mce_batch *batch, const mce_opts *opts,
+                  uint64_t *counts_out, mce_result *agg);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCE_H */

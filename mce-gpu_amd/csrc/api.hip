@@ -1,0 +1,386 @@
+// C ABI of libmce.so: the device pipeline behind include/mce.h.
+//
+// Replaces the reference's single-GPU pipeline (src/main.cu:44-187): upload
+// (main.cu:44-95), degeneracy ordering (kcore.cuh:233-316), orientation
+// (main.cu:102-146) and the persistent enumeration launch with its result
+// read-back (mce/mce.cuh:107-422). Errors are returned as codes; nothing here
+// calls exit().
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "engine.hpp"
+#include "internal.hpp"
+#include "wave.hpp"
+
+struct mce_graph {
+  int device = 0;
+  hipStream_t s = nullptr;
+  mce::DevGraph g;
+  mce::OrderInfo info;
+  bool ordered = false;
+};
+
+namespace mce {
+namespace {
+
+constexpr uint32_t kHistLen = MCE_MAX_OUT_DEGREE + 2;
+
+// Device-side CSR checks: ids in range, no loops, strictly ascending rows,
+// symmetric. A bad graph must never reach the kernels, which index by it.
+__global__ void validate_kernel(uint32_t n, const uint64_t *__restrict__ rp, const uint32_t *__restrict__ ci,
+                                uint32_t *err) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  const uint32_t wid = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  for (uint32_t u = wid; u < n; u += waves) {
+    const uint64_t b = rp[u], e = rp[u + 1];
+    if (e < b) {
+      if (lane_id() == 0) atomicOr(err, 1u);
+      continue;
+    }
+    for (uint64_t k = b + lane_id(); k < e; k += kWave) {
+      const uint32_t w = ci[k];
+      uint32_t bad = 0;
+      if (w >= n) bad = 2;
+      else if (w == u) bad = 4;
+      else if (k > b && ci[k - 1] >= w) bad = 8;
+      else {
+        const uint64_t wb = rp[w], we = rp[w + 1];
+        if (we < wb || we - wb > 0xFFFFFFFFull) bad = 1;
+        else {
+          const uint32_t len = (uint32_t)(we - wb);
+          const uint32_t pos = lower_bound_u32(ci + wb, len, u);
+          if (pos >= len || ci[wb + pos] != u) bad = 16;
+        }
+      }
+      if (bad) atomicOr(err, bad);
+    }
+  }
+}
+
+void fill_order_fields(const mce_graph *h, mce_result *res) {
+  res->degeneracy = h->info.degeneracy;
+  res->max_degree = h->info.max_degree;
+  res->max_out_degree = h->info.max_out_degree;
+}
+
+int check_opts(const mce_opts *o) {
+  if (o->level != MCE_L1 && o->level != MCE_L2) return set_error(MCE_EINVAL, "level must be MCE_L1 or MCE_L2");
+  if (o->induced != MCE_IP && o->induced != MCE_IPX) return set_error(MCE_EINVAL, "induced must be MCE_IP or MCE_IPX");
+  if (o->worklist != MCE_NOWL && o->worklist != MCE_WL)
+    return set_error(MCE_EINVAL, "worklist must be MCE_NOWL or MCE_WL");
+  if (o->task < MCE_TASK_MCE || o->task > MCE_TASK_DONOR_EVAL) return set_error(MCE_EINVAL, "unknown task");
+  // include/config.h:255-264
+  if (o->task == MCE_TASK_BD_EVAL && o->worklist == MCE_NOWL)
+    return set_error(MCE_EINVAL, "No breakdown for kernels without worker list");
+  if (o->task == MCE_TASK_DONOR_EVAL && o->worklist == MCE_NOWL)
+    return set_error(MCE_EINVAL, "No donation for kernels without worker list");
+  if (o->waves_per_group < 0 || o->waves_per_group > 4) return set_error(MCE_EINVAL, "waves_per_group must be 0..4");
+  if (o->groups_per_cu < 0 || o->lds_arena_bytes < 0 || o->hbm_arena_bytes < 0 || o->stack_lds_levels < 0 ||
+      o->donate_min_p < 0 || o->donate_check_nodes < 0)
+    return set_error(MCE_EINVAL, "negative block setting");
+  if (o->lds_arena_bytes && (o->lds_arena_bytes < 1024 || o->lds_arena_bytes % 8))
+    return set_error(MCE_EINVAL, "lds_arena_bytes must be a multiple of 8 and >= 1024");
+  if (o->schedule < MCE_SCHED_PLAIN || o->schedule > MCE_SCHED_GIVEN) return set_error(MCE_EINVAL, "unknown schedule");
+  if (o->schedule == MCE_SCHED_GIVEN && !o->order) return set_error(MCE_EINVAL, "given schedule without an order");
+  return MCE_OK;
+}
+
+}  // namespace
+}  // namespace mce
+
+using namespace mce;
+
+extern "C" {
+
+void mce_opts_default(mce_opts *o) {
+  if (!o) return;
+  memset(o, 0, sizeof(*o));
+  o->level = MCE_L1;
+  o->induced = MCE_IPX;
+  o->worklist = MCE_NOWL;
+  o->task = MCE_TASK_MCE;
+  o->device = 0;
+  o->donate_min_p = 10;
+  o->schedule = MCE_SCHED_PLAIN;
+}
+
+int mce_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int mce_graph_create(const mce_csr *csr, int device, mce_graph **out) {
+  clear_error();
+  if (!csr || !out) return set_error(MCE_EINVAL, "null argument");
+  *out = nullptr;
+  if (csr->n && (!csr->row_ptr || (csr->m && !csr->col_ind))) return set_error(MCE_EINVAL, "null CSR arrays");
+  if (csr->n && (csr->row_ptr[0] != 0 || csr->row_ptr[csr->n] != csr->m))
+    return set_error(MCE_EGRAPH, "row_ptr[0] must be 0 and row_ptr[n] must equal m");
+  for (uint64_t v = 0; v < csr->n; ++v)  // kernels index col_ind by these offsets
+    if (csr->row_ptr[v + 1] < csr->row_ptr[v]) return set_error(MCE_EGRAPH, "row_ptr decreases at vertex %llu", (unsigned long long)v);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(MCE_EDEVICE, "no HIP device available");
+  if (device < 0 || device >= ndev) return set_error(MCE_EINVAL, "device %d out of range (%d devices)", device, ndev);
+  MCE_HIP(hipSetDevice(device));
+  mce_graph *h = new mce_graph();
+  h->device = device;
+  auto fail = [&](int rc) {
+    mce_graph_destroy(h);
+    return rc;
+  };
+  if (hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking) != hipSuccess)
+    return fail(set_error(MCE_EDEVICE, "stream creation failed"));
+  DevGraph &g = h->g;
+  g.n = csr->n;
+  g.m = csr->m;
+  const size_t n1 = (size_t)g.n + 1, m1 = std::max<uint64_t>(g.m, 1);
+  if (hipMalloc(&g.rp, sizeof(uint64_t) * n1) != hipSuccess || hipMalloc(&g.ci, sizeof(uint32_t) * m1) != hipSuccess ||
+      hipMalloc(&g.oc, sizeof(uint32_t) * m1) != hipSuccess ||
+      hipMalloc(&g.split, sizeof(uint64_t) * std::max<size_t>(g.n, 1)) != hipSuccess ||
+      hipMalloc(&g.prio, sizeof(uint32_t) * std::max<size_t>(g.n, 1)) != hipSuccess ||
+      hipMalloc(&g.dplus, sizeof(uint32_t) * std::max<size_t>(g.n, 1)) != hipSuccess)
+    return fail(set_error(MCE_ENOMEM, "device allocation for the graph (n=%u, m=%llu) failed", g.n,
+                          (unsigned long long)g.m));
+  if (g.n) {
+    if (hipMemcpyAsync(g.rp, csr->row_ptr, sizeof(uint64_t) * n1, hipMemcpyHostToDevice, h->s) != hipSuccess ||
+        (g.m && hipMemcpyAsync(g.ci, csr->col_ind, sizeof(uint32_t) * g.m, hipMemcpyHostToDevice, h->s) != hipSuccess))
+      return fail(set_error(MCE_EDEVICE, "graph upload failed"));
+    uint32_t *err = nullptr;
+    if (hipMalloc(&err, sizeof(uint32_t)) != hipSuccess) return fail(set_error(MCE_ENOMEM, "alloc failed"));
+    hipMemsetAsync(err, 0, sizeof(uint32_t), h->s);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(8192, ((uint64_t)g.n * 64 + 255) / 256);
+    validate_kernel<<<std::max<uint32_t>(grid, 1), 256, 0, h->s>>>(g.n, g.rp, g.ci, err);
+    uint32_t herr = 0;
+    hipMemcpyAsync(&herr, err, sizeof(uint32_t), hipMemcpyDeviceToHost, h->s);
+    const hipError_t e = hipStreamSynchronize(h->s);
+    hipFree(err);
+    if (e != hipSuccess) return fail(hip_fail(e, "graph validation"));
+    if (herr) {
+      const char *why = (herr & 2) ? "neighbour id >= n"
+                        : (herr & 1) ? "bad row_ptr"
+                        : (herr & 4) ? "self loop"
+                        : (herr & 8) ? "row not strictly ascending"
+                                     : "not symmetric";
+      return fail(set_error(MCE_EGRAPH, "malformed CSR: %s", why));
+    }
+  }
+  *out = h;
+  return MCE_OK;
+}
+
+void mce_graph_destroy(mce_graph *h) {
+  if (!h) return;
+  hipSetDevice(h->device);
+  if (h->s) hipStreamSynchronize(h->s);
+  hipFree(h->g.rp);
+  hipFree(h->g.ci);
+  hipFree(h->g.oc);
+  hipFree(h->g.split);
+  hipFree(h->g.prio);
+  hipFree(h->g.dplus);
+  if (h->s) hipStreamDestroy(h->s);
+  delete h;
+}
+
+int mce_graph_order(mce_graph *h, mce_result *res) {
+  clear_error();
+  if (!h) return set_error(MCE_EINVAL, "null graph");
+  MCE_HIP(hipSetDevice(h->device));
+  hipEvent_t e0, e1, e2;
+  MCE_HIP(hipEventCreate(&e0));
+  MCE_HIP(hipEventCreate(&e1));
+  MCE_HIP(hipEventCreate(&e2));
+  MCE_HIP(hipEventRecord(e0, h->s));
+  int rc = peel(h->g, h->s, &h->info);
+  if (rc) return rc;
+  MCE_HIP(hipEventRecord(e1, h->s));
+  rc = orient(h->g, h->s, &h->info);
+  if (rc) return rc;
+  MCE_HIP(hipEventRecord(e2, h->s));
+  MCE_HIP(hipEventSynchronize(e2));
+  float t1 = 0, t2 = 0;
+  hipEventElapsedTime(&t1, e0, e1);
+  hipEventElapsedTime(&t2, e1, e2);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipEventDestroy(e2);
+  h->ordered = true;
+  if (res) {
+    fill_order_fields(h, res);
+    res->t_order_ms = t1;
+    res->t_orient_ms = t2;
+  }
+  return MCE_OK;
+}
+
+int mce_graph_download(const mce_graph *h, uint32_t *priority, uint64_t *split, uint32_t *col_ind) {
+  clear_error();
+  if (!h) return set_error(MCE_EINVAL, "null graph");
+  if (!h->ordered) return set_error(MCE_EINVAL, "graph is not ordered yet");
+  MCE_HIP(hipSetDevice(h->device));
+  MCE_HIP(hipStreamSynchronize(h->s));
+  if (priority && h->g.n) MCE_HIP(hipMemcpy(priority, h->g.prio, sizeof(uint32_t) * h->g.n, hipMemcpyDeviceToHost));
+  if (split && h->g.n) MCE_HIP(hipMemcpy(split, h->g.split, sizeof(uint64_t) * h->g.n, hipMemcpyDeviceToHost));
+  if (col_ind && h->g.m) MCE_HIP(hipMemcpy(col_ind, h->g.oc, sizeof(uint32_t) * h->g.m, hipMemcpyDeviceToHost));
+  return MCE_OK;
+}
+
+int mce_graph_count(mce_graph *h, const mce_opts *opts, mce_result *res) {
+  clear_error();
+  if (!h || !opts || !res) return set_error(MCE_EINVAL, "null argument");
+  int rc = check_opts(opts);
+  if (rc) return rc;
+  if (!h->ordered) return set_error(MCE_EINVAL, "graph is not ordered (call mce_graph_order first)");
+  if (opts->device != h->device)
+    return set_error(MCE_EINVAL, "opts.device %d differs from the graph's device %d", opts->device, h->device);
+  if (opts->worklist == MCE_WL) return set_error(MCE_EINVAL, "worker list is not available in this build");
+  MCE_HIP(hipSetDevice(h->device));
+  const DevGraph &g = h->g;
+  hipStream_t s = h->s;
+  fill_order_fields(h, res);
+  res->count = res->hash = res->tree_nodes = res->donations = res->fallback_events = 0;
+  res->max_clique_size = 0;
+  res->wave_stats_len = 0;
+  res->t_count_ms = 0;
+  if (res->hist && res->hist_len) memset(res->hist, 0, sizeof(uint64_t) * res->hist_len);
+
+  uint32_t *order = nullptr;
+  uint64_t *hist = nullptr, *sub_nodes = nullptr, *sub_cliques = nullptr;
+  uint2 *arcs = nullptr;
+  auto cleanup = [&]() {
+    hipFree(order);
+    hipFree(hist);
+    hipFree(sub_nodes);
+    hipFree(sub_cliques);
+    hipFree(arcs);
+  };
+  uint32_t len = 0;
+  if (g.n) {
+    if (hipMalloc(&order, sizeof(uint32_t) * g.n) != hipSuccess) {
+      cleanup();
+      return set_error(MCE_ENOMEM, "order allocation failed");
+    }
+    if (opts->schedule == MCE_SCHED_GIVEN) {
+      std::vector<uint32_t> keep;
+      keep.reserve(opts->order_len);
+      std::vector<uint8_t> seen(g.n, 0);
+      for (uint64_t i = 0; i < opts->order_len; ++i) {
+        const uint32_t v = opts->order[i];
+        if (v >= g.n || seen[v]) {
+          cleanup();
+          return set_error(MCE_EINVAL, "given order has an out-of-range or repeated vertex");
+        }
+        seen[v] = 1;
+        keep.push_back(v);
+      }
+      len = (uint32_t)keep.size();
+      if (len) MCE_HIP(hipMemcpy(order, keep.data(), sizeof(uint32_t) * len, hipMemcpyHostToDevice));
+    } else {
+      // plain: heaviest-first by out-degree
+      rc = schedule_by_key(g, g.dplus, std::max<uint32_t>(h->info.max_out_degree, 1), order, &len, s);
+      if (rc) {
+        cleanup();
+        return rc;
+      }
+    }
+  }
+  const void *items = order;
+  uint64_t n_items = len;
+  if (opts->level == MCE_L2) {
+    rc = arcs_from_order(g, order, len, &arcs, &n_items, s);
+    if (rc) {
+      cleanup();
+      return rc;
+    }
+    items = arcs;
+  }
+  if (hipMalloc(&hist, sizeof(uint64_t) * kHistLen) != hipSuccess) {
+    cleanup();
+    return set_error(MCE_ENOMEM, "histogram allocation failed");
+  }
+  MCE_HIP(hipMemsetAsync(hist, 0, sizeof(uint64_t) * kHistLen, s));
+  if (opts->sub_nodes && g.n) {
+    MCE_HIP(hipMalloc(&sub_nodes, sizeof(uint64_t) * g.n));
+    MCE_HIP(hipMemsetAsync(sub_nodes, 0, sizeof(uint64_t) * g.n, s));
+  }
+  if (opts->sub_cliques && g.n) {
+    MCE_HIP(hipMalloc(&sub_cliques, sizeof(uint64_t) * g.n));
+    MCE_HIP(hipMemsetAsync(sub_cliques, 0, sizeof(uint64_t) * g.n, s));
+  }
+  BkLaunch cfg;
+  cfg.level = opts->level;
+  cfg.ipx = opts->induced == MCE_IPX;
+  cfg.wl = opts->worklist == MCE_WL;
+  cfg.task = opts->task;
+  cfg.waves_per_group = opts->waves_per_group ? (uint32_t)opts->waves_per_group : 4;
+  cfg.lds_bytes = (uint32_t)opts->lds_arena_bytes;
+  cfg.donate_min_p = (uint32_t)opts->donate_min_p;
+  cfg.donate_check_nodes = opts->donate_check_nodes ? (uint32_t)opts->donate_check_nodes : 64;
+  if (opts->groups_per_cu) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    hipGetDevice(&dev);
+    hipGetDeviceProperties(&prop, dev);
+    cfg.groups = (uint32_t)opts->groups_per_cu * (uint32_t)prop.multiProcessorCount;
+  }
+  BkOut out;
+  const bool want_waves = opts->task == MCE_TASK_LB_EVAL || opts->task == MCE_TASK_DONOR_EVAL ||
+                          opts->task == MCE_TASK_BD_EVAL;
+  rc = bk_run(g, h->info, items, n_items, cfg, hist, kHistLen, sub_nodes, sub_cliques,
+              want_waves ? res->wave_stats : nullptr, want_waves ? res->wave_stats_cap : 0, &res->wave_stats_len, s,
+              &out);
+  if (rc) {
+    cleanup();
+    return rc;
+  }
+  std::vector<uint64_t> hh(kHistLen);
+  MCE_HIP(hipMemcpy(hh.data(), hist, sizeof(uint64_t) * kHistLen, hipMemcpyDeviceToHost));
+  if (res->hist && res->hist_len) {
+    const uint32_t k = std::min<uint32_t>(res->hist_len, kHistLen);
+    memcpy(res->hist, hh.data(), sizeof(uint64_t) * k);
+    for (uint32_t i = k; i < kHistLen; ++i) res->hist[k - 1] += hh[i];
+  }
+  if (opts->sub_nodes && g.n) MCE_HIP(hipMemcpy(opts->sub_nodes, sub_nodes, sizeof(uint64_t) * g.n, hipMemcpyDeviceToHost));
+  if (opts->sub_cliques && g.n)
+    MCE_HIP(hipMemcpy(opts->sub_cliques, sub_cliques, sizeof(uint64_t) * g.n, hipMemcpyDeviceToHost));
+  res->count = out.count;
+  res->hash = out.hash;
+  res->tree_nodes = out.nodes;
+  res->donations = out.donations;
+  res->max_clique_size = out.max_size;
+  res->num_waves = out.waves;
+  res->t_count_ms = out.ms;
+  cleanup();
+  return MCE_OK;
+}
+
+int mce_run(const mce_csr *csr, const mce_opts *opts, mce_result *res) {
+  clear_error();
+  if (!csr || !opts || !res) return set_error(MCE_EINVAL, "null argument");
+  int rc = check_opts(opts);
+  if (rc) return rc;
+  mce_graph *h = nullptr;
+  rc = mce_graph_create(csr, opts->device, &h);
+  if (rc) return rc;
+  rc = mce_graph_order(h, res);
+  if (!rc) {
+    const double to = res->t_order_ms, tr = res->t_orient_ms;
+    rc = mce_graph_count(h, opts, res);
+    res->t_order_ms = to;
+    res->t_orient_ms = tr;
+  }
+  mce_graph_destroy(h);
+  return rc;
+}
+
+int mce_run_batch(const mce_batch *batch, const mce_opts *opts, uint64_t *counts_out, mce_result *agg) {
+  clear_error();
+  if (!batch || !opts || !counts_out || !agg) return set_error(MCE_EINVAL, "null argument");
+  return set_error(MCE_EINVAL, "batched engine is not available in this build");
+}
+
+}  // extern "C"

@@ -1,0 +1,84 @@
+// Device-side pipeline of libmce.so: shared structures and the host launchers
+// of the preprocessing (prep.hip) and enumeration (bk.hip) kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mce.h"
+
+namespace mce {
+
+// Graph resident in HBM. Rows of `ci` are the input rows (ascending); rows of
+// `oc` are the same neighbours split as [earlier | later] in degeneracy order,
+// each half ascending; split[v] is the offset where the later half starts.
+struct DevGraph {
+  uint32_t n = 0;
+  uint64_t m = 0;
+  uint64_t *rp = nullptr;
+  uint32_t *ci = nullptr;
+  uint32_t *oc = nullptr;
+  uint64_t *split = nullptr;
+  uint32_t *prio = nullptr;
+  uint32_t *dplus = nullptr;  // out-degree after orientation
+};
+
+struct OrderInfo {
+  uint32_t degeneracy = 0;
+  uint32_t rounds = 0;
+  uint32_t max_degree = 0;
+  uint32_t max_out_degree = 0;
+  uint32_t max_in_degree = 0;  // max |N-(v)|
+};
+
+// Degeneracy ordering by batched k-core peeling (priority = round index).
+int peel(const DevGraph &g, hipStream_t s, OrderInfo *info);
+// Orientation into g.oc / g.split / g.dplus plus degree maxima.
+int orient(DevGraph &g, hipStream_t s, OrderInfo *info);
+
+// First-level work list: vertices with out-degree >= 1, heaviest first by
+// `key` (out-degree for the plain schedule). Returns the list length.
+int schedule_by_key(const DevGraph &g, const uint32_t *key, uint32_t key_max, uint32_t *order,
+                    uint32_t *len_out, hipStream_t s);
+
+struct BkLaunch {
+  int level = 1;       // 1 = vertex subproblems, 2 = arc subproblems
+  bool ipx = true;
+  bool wl = false;
+  int task = 0;        // enum mce_task
+  uint32_t waves_per_group = 4;
+  uint32_t groups = 0; // 0 = fill the device
+  uint32_t lds_bytes = 0;
+  uint64_t hbm_bytes = 0;
+  uint32_t donate_min_p = 10;
+  uint32_t donate_check_nodes = 64;
+};
+
+struct BkOut {
+  uint64_t count = 0, hash = 0, nodes = 0, donations = 0;
+  uint32_t max_size = 0;
+  uint32_t waves = 0;
+  float ms = 0;
+};
+
+// Arc work list for level-2 subproblems: for every vertex u of `order`, its
+// out-arcs (u, w) in order, packed as (u, w) pairs. Returns the arc count.
+int arcs_from_order(const DevGraph &g, const uint32_t *order, uint32_t order_len, uint2 **arcs_out,
+                    uint64_t *n_arcs, hipStream_t s);
+
+// Enumerates maximal cliques over a first-level work list.
+// level 1: items are vertex ids (uint32); level 2: items are arcs (uint2).
+// hist (device, hist_len entries) is accumulated.
+int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64_t n_items,
+           const BkLaunch &cfg, uint64_t *d_hist, uint32_t hist_len, uint64_t *d_sub_nodes,
+           uint64_t *d_sub_cliques, mce_wave_stat *wave_stats, uint32_t wave_stats_cap,
+           uint32_t *wave_stats_len, hipStream_t s, BkOut *out);
+
+int hip_fail(hipError_t e, const char *what);
+
+}  // namespace mce
+
+#define MCE_HIP(call)                                              \
+  do {                                                             \
+    hipError_t e_ = (call);                                        \
+    if (e_ != hipSuccess) return ::mce::hip_fail(e_, #call);       \
+  } while (0)

@@ -1,0 +1,121 @@
+"""GPU engine vs CPU oracle: bit-exact count, size histogram, clique hash and
+BK tree-node count, through the C ABI, for every -p/-i variant."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = [(lvl, ind) for lvl in (1, 2) for ind in (1, 2)]  # (L1|L2, IP|IPX)
+
+
+def rand_graph(mce, n, p, seed):
+    rng = np.random.default_rng(seed)
+    iu = np.triu_indices(n, 1)
+    keep = rng.random(iu[0].size) < p
+    return mce.from_edges(iu[0][keep], iu[1][keep], n=n)
+
+
+def assert_same(res, ref, what, nodes=True):
+    assert res["count"] == ref["count"], (what, res["count"], ref["count"])
+    assert res["hist"] == ref["hist"], (what, res["hist"], ref["hist"])
+    assert res["hash"] == ref["hash"], what
+    if nodes:
+        assert res["tree_nodes"] == ref["tree_nodes"], (what, res["tree_nodes"], ref["tree_nodes"])
+
+
+def run_all(mce, oracle, g, what, nodes=True, **block):
+    with mce.DeviceGraph(g) as dg:
+        info = dg.order()
+        for lvl, ind in VARIANTS:
+            ref = oracle.mce(g, level=lvl, ipx=(ind == 2))
+            res = dg.count(level=lvl, induced=ind, **block)
+            assert info["degeneracy"] == ref["degeneracy"], what
+            assert_same(res, ref, (what, lvl, ind), nodes=nodes)
+
+
+def test_random_small_graphs(mce, oracle):
+    for seed in range(60):
+        rng = np.random.default_rng(1000 + seed)
+        n = int(rng.integers(1, 90))
+        p = float(rng.random())
+        print(f"case seed={seed} n={n} p={p:.3f}", flush=True)
+        run_all(mce, oracle, rand_graph(mce, n, p, seed), f"rand-{seed}")
+
+
+def test_ordering_and_orientation_match_oracle(mce, oracle):
+    for seed in range(8):
+        g = rand_graph(mce, 300, 0.05 + 0.05 * seed, seed)
+        prio_o, split_o, oc_o, d_o = oracle.order_orient(g)
+        with mce.DeviceGraph(g) as dg:
+            info = dg.order()
+            prio, split, oc = dg.download()
+        assert info["degeneracy"] == d_o
+        np.testing.assert_array_equal(prio, prio_o)
+        np.testing.assert_array_equal(split, split_o)
+        np.testing.assert_array_equal(oc, oc_o)
+
+
+def test_er_4096_config(mce, oracle):
+    """Erdős–Rényi n=4096, m=65,536, seed 1 (the small plain-method config)."""
+    g = mce.gen_er(4096, 65536, 1)
+    run_all(mce, oracle, g, "er4096")
+
+
+def test_rmat_scale12(mce, oracle):
+    # R-MAT (.57, .19, .19, .05), edge factor 16: 0.57M maximal cliques at scale 12
+    g = mce.gen_rmat(12, 16, seed=3)
+    run_all(mce, oracle, g, "rmat12")
+
+
+def test_multiword_bitmaps(mce, oracle):
+    # dense graphs: out-degrees above 64 and 128 exercise W = 2..4 words
+    for seed, (n, p) in enumerate([(200, 0.5), (400, 0.6), (160, 0.9)]):
+        print(f"case dense n={n} p={p}", flush=True)
+        run_all(mce, oracle, rand_graph(mce, n, p, 50 + seed), f"dense-{n}")
+
+
+def test_hbm_arena_and_frame_spill(mce, oracle):
+    # a tiny LDS arena forces the HBM arena and HBM stack frames
+    g = rand_graph(mce, 220, 0.55, 7)
+    run_all(mce, oracle, g, "hbm-arena", lds_arena_bytes=1024)
+    # a complete graph K_150: one clique, 149-deep recursion in one subproblem
+    n = 150
+    iu = np.triu_indices(n, 1)
+    k = mce.from_edges(iu[0], iu[1], n=n)
+    run_all(mce, oracle, k, "K150", lds_arena_bytes=2048)
+
+
+def test_degenerate_graphs(mce, oracle):
+    # isolated vertices, a single edge, a star, a path, an empty graph
+    cases = {
+        "edge": ([0], [1], 2),
+        "isolated": ([0, 5], [1, 6], 9),
+        "star": ([0] * 9, list(range(1, 10)), 10),
+        "path": (list(range(0, 30)), list(range(1, 31)), 31),
+        "empty": ([], [], 5),
+    }
+    for name, (s, d, n) in cases.items():
+        g = mce.from_edges(np.array(s, np.uint32), np.array(d, np.uint32), n=n)
+        run_all(mce, oracle, g, name)
+
+
+def test_schedule_invariance(mce, oracle):
+    """Any order of first-level subproblems gives identical results."""
+    g = mce.gen_rmat(12, 16, seed=5)
+    ref = oracle.mce(g, level=1, ipx=True)
+    rng = np.random.default_rng(9)
+    with mce.DeviceGraph(g) as dg:
+        dg.order()
+        for t in range(3):
+            order = rng.permutation(g.n).astype(np.uint32)
+            res = dg.count(level=1, induced=2, order=order)
+            assert_same(res, ref, f"perm-{t}")
+
+
+def test_per_subproblem_labels(mce, oracle):
+    g = mce.gen_rmat(12, 16, seed=11)
+    ref = oracle.mce(g, level=1, ipx=True, sub_nodes=True, sub_cliques=True)
+    res = mce.run(g, level=1, induced=2, sub_nodes=True, sub_cliques=True)
+    np.testing.assert_array_equal(res["sub_nodes"], ref["sub_nodes"])
+    np.testing.assert_array_equal(res["sub_cliques"], ref["sub_cliques"])
+    assert int(res["sub_cliques"].sum()) == res["count"]
