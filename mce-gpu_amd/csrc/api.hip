@@ -237,7 +237,6 @@ int mce_graph_count(mce_graph *h, const mce_opts *opts, mce_result *res) {
   if (!h->ordered) return set_error(MCE_EINVAL, "graph is not ordered (call mce_graph_order first)");
   if (opts->device != h->device)
     return set_error(MCE_EINVAL, "opts.device %d differs from the graph's device %d", opts->device, h->device);
-  if (opts->worklist == MCE_WL) return set_error(MCE_EINVAL, "worker list is not available in this build");
   MCE_HIP(hipSetDevice(h->device));
   const DevGraph &g = h->g;
   hipStream_t s = h->s;

@@ -1,4 +1,5 @@
-// Bron-Kerbosch with pivoting on gfx950: one wave64 per first-level subproblem.
+// Bron-Kerbosch with pivoting on gfx950: one wave64 per subproblem, with an
+// optional worker list that moves subtrees from busy waves to idle ones.
 //
 // Semantics follow the reference (yen-hsiang-chang/mce-gpu):
 //   L1 subproblem of vertex v: R = {v}, P = N+(v), X = N-(v)
@@ -10,18 +11,33 @@
 //   the branch (mce/mce_utils.cuh:338-472); a leaf with P and X empty is a
 //   maximal clique (mce/mce_utils.cuh:507-524); IP tests outside-X vertices
 //   by binary search of N-(c) (mce/mce_utils.cuh:265-292), IPX by a bit of
-//   their bitmap row (mce/mce_utils.cuh:294-319).
+//   their bitmap row (mce/mce_utils.cuh:294-319);
+//   worker list: idle workers receive subtrees from busy ones once the
+//   first-level work is gone (mce/mce_utils.cuh:241-263, :861-1217).
 //
-// Design (not the reference's): the induced subgraph over S = P0 is a p x W
-// bit matrix of 64-bit words (W = ceil(p/64)); set bitmaps P, X and the
-// branch mask live one word per lane (lane k holds word k) in registers, so
-// intersections, emptiness tests and next-candidate search are single wave
-// instructions plus a ballot, with no workgroup barriers on the hot loop.
-// Pivot scores run one candidate row per lane. The per-wave arena (S, the
-// matrix, the outside-X rows and lists) sits in LDS when it fits the bitmap
-// cutoff and in a per-wave HBM arena otherwise; stack frames beyond the LDS
-// budget spill to HBM. Tie-breaks are deterministic (lowest index), so the
-// tree is reproducible and matches the CPU oracle node for node.
+// Design (not the reference's): the induced subgraph over a sorted vertex set
+// S is a |S| x W bit matrix of 64-bit words (W = ceil(|S|/64)); the sets P, X
+// and the branch mask live one word per lane (lane k holds word k) in
+// registers, so intersections, emptiness tests and next-candidate search are
+// single wave instructions plus a ballot, with no workgroup barriers on the
+// hot loop. Pivot scores run one candidate row per lane. The per-wave arena
+// (S, the matrix, the outside-X rows and lists) sits in LDS when it fits the
+// bitmap cutoff and in a per-wave HBM arena otherwise; stack frames beyond
+// the LDS budget spill to HBM. Tie-breaks are deterministic (lowest index,
+// outside-X rows by vertex id), so the tree is reproducible, matches the CPU
+// oracle node for node, and a donated subtree expands exactly as it would
+// have in the donor.
+//
+// Worker list: a bounded lock-free ring of task slots in HBM (per-slot
+// sequence numbers, agent-scope acquire/release so a slot written on one XCD
+// is seen on another). When idle waves outnumber queued tasks, a busy wave
+// takes the next candidate of its shallowest frame (the largest remaining
+// subtree) and publishes that child: its P and X words, the donor's vertex
+// set and bit matrix, and the outside-X entries that survive. The receiver
+// copies them into its own arena and continues in the donor's index space,
+// so no arena is ever shared and nothing is rebuilt. A counter of outstanding work
+// (tickets + published tasks) ends the kernel exactly; a realtime watchdog
+// turns any termination bug into an error flag instead of a hang.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,7 +52,31 @@ namespace mce {
 
 namespace {
 
-enum Acc : uint32_t { kAccCount = 0, kAccHash, kAccNodes, kAccDonations, kAccOverflow, kAccMaxSize, kAccN };
+enum Acc : uint32_t {
+  kAccCount = 0,
+  kAccHash,
+  kAccNodes,
+  kAccDonations,
+  kAccOverflow,
+  kAccMaxSize,
+  kAccError,
+  kAccN
+};
+
+enum Stat : int { kStatNone = 0, kStatLb = 1, kStatDonor = 2, kStatBd = 3 };
+enum Bd : int { kBdOther = 0, kBdInduced, kBdQueue, kBdPivot, kBdSetOps, kBdMaximal, kBdBranch };
+
+// Ring control words, each on its own 64-byte line.
+constexpr uint32_t kCtlHead = 0, kCtlTail = 8, kCtlOutstanding = 16, kCtlIdle = 24, kCtlWords = 32;
+constexpr unsigned long long kNoSlot = ~0ull;
+constexpr uint32_t kSlotHeader = 64;  // bytes
+// IPX subproblems with at most this many outside-X rows keep them as a bitmap
+// over row indices (one word per lane, like P and X); larger ones use lists.
+constexpr uint32_t kMaxXRows = 4096;
+enum TaskKind : uint32_t { kTaskList = 0, kTaskBits = 1 };
+// 100 MHz realtime ticks an idle wave waits without receiving work before it
+// reports a termination failure and exits: 120 s.
+constexpr uint64_t kWatchdogTicks = 12000000000ull;
 
 struct BkParams {
   const uint64_t *rp;
@@ -46,23 +86,36 @@ struct BkParams {
   const void *items;
   uint64_t n_items;
   uint32_t *ticket;
-  uint64_t t1;      // tickets below t1 carry one item each
-  uint32_t chunk;   // later tickets carry `chunk` items
+  uint64_t t1;     // tickets below t1 carry one item each
+  uint32_t chunk;  // later tickets carry `chunk` items
   uint32_t level;
-  uint32_t lds_bytes;   // per-wave LDS arena
-  uint8_t *hbm;         // per-wave HBM arenas
+  uint32_t lds_bytes;  // per-wave LDS arena
+  uint8_t *hbm;        // per-wave HBM arenas
   uint64_t hbm_bytes;
   unsigned long long *acc;
   unsigned long long *hist;
   uint32_t hist_len;
-  uint32_t hist_lds;    // histogram bins cached in LDS per workgroup
+  uint32_t hist_lds;  // histogram bins cached in LDS per workgroup
   unsigned long long *sub_nodes;
   unsigned long long *sub_cliques;
-  uint64_t *overflow;   // items that need a larger arena
+  uint64_t *overflow;  // items that need a larger arena
   uint64_t overflow_cap;
-  unsigned long long *wave_val;  // per-wave statistic (lb)
+  unsigned long long *wave_val;  // per-wave statistic
   uint32_t *wave_cu;
-  uint32_t wpg;         // waves per workgroup
+  unsigned long long *wave_bd;  // 7 cycle counters per wave
+  uint32_t wpg;                 // waves per workgroup
+  // worker list
+  unsigned long long *ctl;
+  unsigned long long *seq;
+  uint8_t *ring;
+  uint32_t ring_cap;
+  uint64_t slot_bytes;
+  uint32_t slot_pmax;  // largest donor |S|
+  uint32_t slot_wmax;
+  uint32_t slot_xcap;  // largest outside-X list carried
+  uint32_t slot_oP, slot_oS, slot_oA, slot_oXI, slot_oXR;  // byte offsets in a slot
+  uint32_t donate_min_p;
+  uint32_t donate_check;
 };
 
 struct Sub {
@@ -71,23 +124,49 @@ struct Sub {
   uint64_t *XR;
   uint32_t *XL;
   uint32_t *XA;
+  uint32_t *XV;  // IPX: vertex id of each outside-X row
   uint8_t *fr_lds;
   uint8_t *fr_hbm;
   uint32_t lds_levels;
   uint32_t FS;
   uint32_t p, W, q, xe0;
+  uint32_t WX;  // bitmap path: words of the outside-X row bitmap
   uint32_t rsize;
+  uint32_t root;  // first-level vertex the subproblem belongs to
   uint64_t rmix;
-  // IP: N-(c) lookups
   const uint32_t *oc;
   const uint64_t *rp;
   const uint64_t *split;
 };
 
 struct WaveAcc {
-  uint64_t count = 0, hash = 0, nodes = 0;
+  uint64_t count = 0, hash = 0, nodes = 0, donations = 0;
   uint32_t max_size = 0;
   uint64_t sub_cliques = 0;
+  int32_t budget = 0;
+  uint64_t last_poll = 0;
+};
+
+template <int kStat>
+struct Clock {
+  uint64_t acc[7];
+  uint64_t last;
+  int cur;
+  __device__ __forceinline__ void start() {
+    if constexpr (kStat == kStatBd) {
+      for (int i = 0; i < 7; ++i) acc[i] = 0;
+      last = __builtin_amdgcn_s_memtime();
+      cur = kBdOther;
+    }
+  }
+  __device__ __forceinline__ void to(int c) {
+    if constexpr (kStat == kStatBd) {
+      const uint64_t t = __builtin_amdgcn_s_memtime();
+      acc[cur] += t - last;
+      last = t;
+      cur = c;
+    }
+  }
 };
 
 __device__ __forceinline__ uint64_t align8(uint64_t x) { return (x + 7) & ~7ull; }
@@ -98,12 +177,12 @@ __device__ __forceinline__ uint8_t *frame_at(const Sub &s, uint32_t d) {
 
 // Fixed arena bytes for bounds (p, q, W).
 __device__ __forceinline__ uint64_t fixed_bytes(uint32_t p, uint32_t q, uint32_t W, bool ipx) {
-  return align8(4ull * p) + 8ull * p * W + (ipx ? 8ull * q * W : 0) + 2 * align8(4ull * q);
+  return align8(4ull * p) + 8ull * p * W + (ipx ? 8ull * q * W + align8(4ull * q) : 0) + 2 * align8(4ull * q);
 }
 
 // Lays out the arena for bounds (p, q, W). Returns false when the HBM arena
 // is too small.
-__device__ bool place(Sub &s, uint32_t p, uint32_t q, uint32_t W, bool ipx, uint8_t *lds, uint32_t lds_bytes,
+__device__ __forceinline__ bool place(Sub &s, uint32_t p, uint32_t q, uint32_t W, bool ipx, uint8_t *lds, uint32_t lds_bytes,
                       uint8_t *hbm, uint64_t hbm_bytes) {
   const uint64_t fixed = fixed_bytes(p, q, W, ipx);
   const uint32_t FS = 16 * W + 16;
@@ -132,6 +211,8 @@ __device__ bool place(Sub &s, uint32_t p, uint32_t q, uint32_t W, bool ipx, uint
   o += 8ull * p * W;
   s.XR = (uint64_t *)(base + o);
   if (ipx) o += 8ull * q * W;
+  s.XV = (uint32_t *)(base + o);
+  if (ipx) o += align8(4ull * q);
   s.XL = (uint32_t *)(base + o);
   o += align8(4ull * q);
   s.XA = (uint32_t *)(base + o);
@@ -155,13 +236,14 @@ __device__ __forceinline__ const uint64_t *row_of(const Sub &s, uint32_t piv) {
   return piv < s.p ? s.A + (uint64_t)piv * s.W : s.XR + (uint64_t)(piv - s.p) * s.W;
 }
 
-// Pivot: max |P & N(u)| over u in P|X (bit order), then outside-X rows (IPX);
-// ties go to the lowest index (outside rows are indexed p + row). Uniform out.
+// Pivot: max |P & N(u)| over u in P|X (bit order), then outside-X rows (IPX).
+// Ties go to the lowest index, and every inside vertex precedes every outside
+// row; outside rows tie-break by vertex id. Uniform outputs.
 template <bool kIPX>
 __device__ __forceinline__ void pick_pivot(const Sub &s, uint64_t Pw, uint64_t Xw, uint32_t xe_len, uint32_t &piv,
                                            uint32_t &best) {
   const uint32_t L = lane_id(), W = s.W;
-  uint32_t bs = 0, bi = kNone;
+  uint32_t bs = 0, bk = kNone, brow = 0;
   const uint64_t C = Pw | Xw;
   for (uint32_t k = 0; k < W; ++k) {
     const uint64_t ck = rdlane64(C, k);
@@ -171,9 +253,9 @@ __device__ __forceinline__ void pick_pivot(const Sub &s, uint64_t Pw, uint64_t X
       const uint64_t *row = s.A + (uint64_t)u * W;
       uint32_t sc = 0;
       for (uint32_t j = 0; j < W; ++j) sc += popc64(rdlane64(Pw, j) & row[j]);
-      if (bi == kNone || sc > bs) {
+      if (bk == kNone || sc > bs) {
         bs = sc;
-        bi = u;
+        bk = u;
       }
     }
   }
@@ -185,18 +267,26 @@ __device__ __forceinline__ void pick_pivot(const Sub &s, uint64_t Pw, uint64_t X
         const uint64_t *row = s.XR + (uint64_t)e * W;
         uint32_t sc = 0;
         for (uint32_t j = 0; j < W; ++j) sc += popc64(rdlane64(Pw, j) & row[j]);
-        const uint32_t idx = s.p + e;
-        if (bi == kNone || sc > bs || (sc == bs && idx < bi)) {
+        const uint32_t key = 0x80000000u | s.XV[e];
+        if (bk == kNone || sc > bs || (sc == bs && key < bk)) {
           bs = sc;
-          bi = idx;
+          bk = key;
+          brow = e;
         }
       }
     }
   }
-  const uint32_t key = bi == kNone ? 0u : bs + 1u;
-  const uint32_t mx = wave_max(key);
-  piv = wave_min((bi != kNone && bs + 1u == mx) ? bi : kNone);
+  const uint32_t mx = wave_max(bk == kNone ? 0u : bs + 1u);
+  const uint32_t kmin = wave_min((bk != kNone && bs + 1u == mx) ? bk : kNone);
   best = mx - 1u;
+  if (kmin == kNone) {
+    piv = kNone;
+  } else if (kmin & 0x80000000u) {
+    const uint32_t who = ctz64(ballot(bk == kmin));
+    piv = s.p + rdlane(brow, who);
+  } else {
+    piv = kmin;
+  }
 }
 
 // Is outside-X entry e adjacent to S[c]? IPX: bit c of its row. IP: e is a
@@ -209,15 +299,22 @@ __device__ __forceinline__ bool xe_adj(const Sub &s, uint32_t e, uint32_t c, con
 }
 
 template <bool kIPX>
-__device__ __forceinline__ bool xe_any(const Sub &s, uint32_t len, uint32_t c) {
-  const uint32_t *nb = nullptr;
-  uint32_t nlen = 0;
+__device__ __forceinline__ void in_nbrs(const Sub &s, uint32_t c, const uint32_t *&nb, uint32_t &nlen) {
+  nb = nullptr;
+  nlen = 0;
   if (!kIPX) {
     const uint32_t cv = s.S[c];
     const uint64_t b = s.rp[cv];
     nb = s.oc + b;
     nlen = (uint32_t)(s.split[cv] - b);
   }
+}
+
+template <bool kIPX>
+__device__ __forceinline__ bool xe_any(const Sub &s, uint32_t len, uint32_t c) {
+  const uint32_t *nb;
+  uint32_t nlen;
+  in_nbrs<kIPX>(s, c, nb, nlen);
   for (uint32_t b = 0; b < len; b += kWave) {
     const uint32_t i = b + lane_id();
     const bool t = i < len && xe_adj<kIPX>(s, s.XL[i], c, nb, nlen);
@@ -231,14 +328,9 @@ __device__ __forceinline__ bool xe_any(const Sub &s, uint32_t len, uint32_t c) {
 // intact after the child returns.
 template <bool kIPX>
 __device__ __forceinline__ uint32_t xe_partition(const Sub &s, uint32_t len, uint32_t c) {
-  const uint32_t *nb = nullptr;
-  uint32_t nlen = 0;
-  if (!kIPX) {
-    const uint32_t cv = s.S[c];
-    const uint64_t b = s.rp[cv];
-    nb = s.oc + b;
-    nlen = (uint32_t)(s.split[cv] - b);
-  }
+  const uint32_t *nb;
+  uint32_t nlen;
+  in_nbrs<kIPX>(s, c, nb, nlen);
   const uint32_t L = lane_id();
   uint32_t kept = 0, drop = 0;
   for (uint32_t b = 0; b < len; b += kWave) {
@@ -269,15 +361,15 @@ __device__ __forceinline__ void report(const BkParams &pr, unsigned long long *h
   }
 }
 
-// Zeroes n 64-bit words.
 __device__ __forceinline__ void zero_words(uint64_t *a, uint64_t n) {
   for (uint64_t i = lane_id(); i < n; i += kWave) a[i] = 0;
 }
 
 // Builds the induced bit matrix over S and (IPX) the outside-X rows; fills
-// s.xe0. XL holds the outside-X vertex ids on entry.
+// s.xe0. XL holds the outside-X vertex ids on entry; with IPX it holds row
+// indices on exit and XV keeps the ids.
 template <bool kIPX>
-__device__ void build(const BkParams &pr, Sub &s) {
+__device__ __forceinline__ void build(const BkParams &pr, Sub &s) {
   const uint32_t L = lane_id(), p = s.p, W = s.W;
   zero_words(s.A, (uint64_t)p * W);
   if (kIPX) zero_words(s.XR, (uint64_t)s.q * W);
@@ -305,6 +397,7 @@ __device__ void build(const BkParams &pr, Sub &s) {
       bool any = false;
       if (i < s.q) {
         const uint32_t x = s.XL[i];
+        s.XV[i] = x;
         uint64_t *row = s.XR + (uint64_t)i * W;
         for (uint64_t k = pr.split[x]; k < pr.rp[x + 1]; ++k) {
           const uint32_t y = pr.oc[k];
@@ -326,25 +419,218 @@ __device__ void build(const BkParams &pr, Sub &s) {
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
 }
 
+// ---- worker-list ring ---------------------------------------------------
+
+__device__ __forceinline__ unsigned long long ld_relaxed(unsigned long long *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Claims a slot to publish into (lane 0 only). kNoSlot when full or contended.
+__device__ __forceinline__ unsigned long long ring_claim_push(const BkParams &pr) {
+  unsigned long long *tail = pr.ctl + kCtlTail;
+  unsigned long long pos = ld_relaxed(tail);
+  for (int it = 0; it < 64; ++it) {
+    const unsigned long long sq =
+        __hip_atomic_load(&pr.seq[pos % pr.ring_cap], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const long long dif = (long long)(sq - pos);
+    if (dif == 0) {
+      if (__hip_atomic_compare_exchange_weak(tail, &pos, pos + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT))
+        return pos;
+    } else if (dif < 0) {
+      return kNoSlot;
+    } else {
+      pos = ld_relaxed(tail);
+    }
+  }
+  return kNoSlot;
+}
+
+// Claims a published slot to consume (lane 0 only). kNoSlot when empty.
+__device__ __forceinline__ unsigned long long ring_claim_pop(const BkParams &pr) {
+  unsigned long long *head = pr.ctl + kCtlHead;
+  unsigned long long pos = ld_relaxed(head);
+  for (int it = 0; it < 64; ++it) {
+    const unsigned long long sq =
+        __hip_atomic_load(&pr.seq[pos % pr.ring_cap], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const long long dif = (long long)(sq - (pos + 1));
+    if (dif == 0) {
+      if (__hip_atomic_compare_exchange_weak(head, &pos, pos + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT))
+        return pos;
+    } else if (dif < 0) {
+      return kNoSlot;
+    } else {
+      pos = ld_relaxed(head);
+    }
+  }
+  return kNoSlot;
+}
+
+// Idle waves minus queued tasks, sampled by lane 0.
+__device__ __forceinline__ int32_t donation_budget(const BkParams &pr) {
+  int32_t b = 0;
+  if (lane_id() == 0) {
+    const unsigned long long idle = ld_relaxed(pr.ctl + kCtlIdle);
+    const unsigned long long h = ld_relaxed(pr.ctl + kCtlHead), t = ld_relaxed(pr.ctl + kCtlTail);
+    b = (int32_t)idle - (int32_t)(t > h ? t - h : 0);
+  }
+  return (int32_t)uniform((uint32_t)b);
+}
+
+// Publishes the next child of the shallowest frame that still has one, when
+// that child is large enough. The task carries the donor's induced bit matrix
+// (same index space), the child's P and X words and its outside-X entries
+// (ids, and rows with IPX), so the receiver copies instead of rebuilding.
+// The frame is updated as if the child had been explored. Returns true on a
+// donation.
 template <bool kIPX>
-__device__ void bk_solve(const BkParams &pr, Sub &s, unsigned long long *hist_lds, WaveAcc &wa) {
-  const uint32_t L = lane_id(), W = s.W, p = s.p;
+__device__ __forceinline__ bool donate_shallowest(const BkParams &pr, Sub &s, uint32_t d, uint64_t &cP, uint64_t &cX, uint32_t cpiv,
+                                  uint32_t cxe, uint64_t cmix, WaveAcc &wa) {
+  const uint32_t L = lane_id(), W = s.W;
   const bool act = L < W;
-  uint64_t cP = 0, cX = 0;
-  if (act) cP = (L == W - 1 && (p & 63)) ? ((1ull << (p & 63)) - 1) : ~0ull;
+  if (s.p > pr.slot_pmax) return false;
+  for (uint32_t j = 0; j <= d; ++j) {
+    uint64_t Pj = cP, Xj = cX, mixj = cmix;
+    uint32_t pivj = cpiv, xej = cxe;
+    uint8_t *f = nullptr;
+    if (j < d) {
+      f = frame_at(s, j);
+      if (act) {
+        Pj = ((const uint64_t *)f)[L];
+        Xj = ((const uint64_t *)f)[W + L];
+      } else {
+        Pj = Xj = 0;
+      }
+      const uint32_t *meta = (const uint32_t *)(f + 16 * W);
+      pivj = uniform(meta[0]);
+      xej = uniform(meta[1]);
+      mixj = uniform64(*(const uint64_t *)(f + 16 * W + 8));
+    }
+    const uint64_t Nj = act ? row_of(s, pivj)[L] : 0;
+    const uint64_t B = Pj & ~Nj;
+    const uint64_t bm = ballot(B != 0);
+    if (bm == 0) continue;
+    const uint32_t k0 = ctz64(bm);
+    const uint32_t bit = ctz64(rdlane64(B, k0));
+    const uint32_t c = k0 * 64 + bit;
+    const uint64_t Ac = act ? s.A[(uint64_t)c * W + L] : 0;
+    const uint64_t Pn = Pj & Ac, Xn = Xj & Ac;
+    if (wave_sum(popc64(Pn)) < pr.donate_min_p) continue;
+    // outside X of the child: entries of the frame's list adjacent to S[c]
+    const uint32_t *nb;
+    uint32_t nlen;
+    in_nbrs<kIPX>(s, c, nb, nlen);
+    uint32_t nx = 0;
+    for (uint32_t b = 0; b < xej; b += kWave) {
+      const uint32_t i = b + L;
+      nx += popc64(ballot(i < xej && xe_adj<kIPX>(s, s.XL[i], c, nb, nlen)));
+    }
+    if (nx > pr.slot_xcap || hbm_need(s.p, nx, kIPX, pr.lds_bytes) > pr.hbm_bytes) continue;
+    unsigned long long pos = 0;
+    if (L == 0) {
+      pos = ring_claim_push(pr);
+      if (pos != kNoSlot) atomicAdd(pr.ctl + kCtlOutstanding, 1ull);
+    }
+    pos = uniform64(pos);
+    if (pos == kNoSlot) return false;
+    uint8_t *slot = pr.ring + (pos % pr.ring_cap) * (uint64_t)pr.slot_bytes;
+    if (L == 0) {
+      uint32_t *h = (uint32_t *)slot;
+      h[0] = s.root;
+      h[1] = s.rsize + j + 1;
+      h[2] = s.p;
+      h[3] = W;
+      h[4] = nx;
+      h[5] = kTaskList;
+      *(uint64_t *)(slot + 24) = mixj + splitmix64(s.S[c]);
+    }
+    uint64_t *sp = (uint64_t *)(slot + pr.slot_oP);
+    if (act) {
+      sp[L] = Pn;
+      sp[pr.slot_wmax + L] = Xn;
+    }
+    uint32_t *sS = (uint32_t *)(slot + pr.slot_oS);
+    for (uint32_t i = L; i < s.p; i += kWave) sS[i] = s.S[i];
+    uint64_t *sA = (uint64_t *)(slot + pr.slot_oA);
+    const uint64_t nA = (uint64_t)s.p * W;
+    for (uint64_t i = L; i < nA; i += kWave) sA[i] = s.A[i];
+    uint32_t *sXI = (uint32_t *)(slot + pr.slot_oXI);
+    uint64_t *sXR = (uint64_t *)(slot + pr.slot_oXR);
+    uint32_t w = 0;
+    for (uint32_t b = 0; b < xej; b += kWave) {
+      const uint32_t i = b + L;
+      uint32_t e = 0;
+      bool t = false;
+      if (i < xej) {
+        e = s.XL[i];
+        t = xe_adj<kIPX>(s, e, c, nb, nlen);
+      }
+      const uint64_t m = ballot(t);
+      if (t) {
+        const uint32_t k = w + mbcnt(m);
+        if (kIPX) {
+          sXI[k] = s.XV[e];
+          for (uint32_t x = 0; x < W; ++x) sXR[(uint64_t)k * W + x] = s.XR[(uint64_t)e * W + x];
+        } else {
+          sXI[k] = e;
+        }
+      }
+      w += popc64(m);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (L == 0) __hip_atomic_store(&pr.seq[pos % pr.ring_cap], pos + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    // the donor treats c as explored: P -> X in frame j
+    const uint64_t m = 1ull << bit;
+    if (j == d) {
+      if (L == k0) {
+        cP &= ~m;
+        cX |= m;
+      }
+    } else if (L == k0) {
+      ((uint64_t *)f)[L] = Pj & ~m;
+      ((uint64_t *)f)[W + L] = Xj | m;
+    }
+    wa.nodes++;
+    wa.donations++;
+    return true;
+  }
+  return false;
+}
+
+// Runs BK from the root state (P, X) of an arena that `build` has filled.
+template <bool kIPX, bool kWL, int kStat>
+__device__ __forceinline__ void bk_solve(const BkParams &pr, Sub &s, uint64_t cP, uint64_t cX, unsigned long long *hist_lds,
+                         WaveAcc &wa, Clock<kStat> &clk) {
+  const uint32_t L = lane_id(), W = s.W;
+  const bool act = L < W;
   uint32_t xe = s.xe0;
   uint32_t piv, best;
+  clk.to(kBdPivot);
   pick_pivot<kIPX>(s, cP, cX, xe, piv, best);
-  if (piv == kNone || best == p) return;  // an outside X vertex covers all of P
+  if (piv == kNone || best == wave_sum(popc64(cP))) return;  // an X vertex covers all of P
   uint64_t cN = act ? row_of(s, piv)[L] : 0;
   uint64_t mix = s.rmix;
   uint32_t d = 0;
-  uint64_t nodes = 0;
   while (true) {
+    if constexpr (kWL) {
+      if (wa.nodes - wa.last_poll >= pr.donate_check) {
+        wa.last_poll = wa.nodes;
+        clk.to(kBdQueue);
+        wa.budget = donation_budget(pr);
+      }
+      if (wa.budget > 0) {
+        clk.to(kBdQueue);
+        if (donate_shallowest<kIPX>(pr, s, d, cP, cX, piv, xe, mix, wa)) wa.budget--;
+        else wa.budget = 0;
+      }
+    }
+    clk.to(kBdSetOps);
     const uint64_t B = cP & ~cN;
     const uint64_t bm = ballot(B != 0);
     if (bm == 0) {
       if (d == 0) break;
+      clk.to(kBdBranch);
       --d;
       const uint8_t *f = frame_at(s, d);
       if (act) {
@@ -367,16 +653,19 @@ __device__ void bk_solve(const BkParams &pr, Sub &s, unsigned long long *hist_ld
     }
     const uint64_t Ac = act ? s.A[(uint64_t)c * W + L] : 0;
     const uint64_t Pn = cP & Ac, Xn = cX & Ac;
-    ++nodes;
+    wa.nodes++;
     const uint64_t cmix = mix + splitmix64(uniform(s.S[c]));
     if (ballot(Pn != 0) == 0) {
+      clk.to(kBdMaximal);
       if (ballot(Xn != 0) == 0 && (xe == 0 || !xe_any<kIPX>(s, xe, c))) report(pr, hist_lds, wa, s.rsize + d + 1, cmix);
       continue;
     }
     const uint32_t xn = xe ? xe_partition<kIPX>(s, xe, c) : 0;
     uint32_t npiv, nbest;
+    clk.to(kBdPivot);
     pick_pivot<kIPX>(s, Pn, Xn, xn, npiv, nbest);
     if (npiv == kNone || nbest == wave_sum(popc64(Pn))) continue;  // an X vertex covers P
+    clk.to(kBdBranch);
     uint8_t *f = frame_at(s, d);
     if (act) {
       ((uint64_t *)f)[L] = cP;
@@ -394,12 +683,350 @@ __device__ void bk_solve(const BkParams &pr, Sub &s, unsigned long long *hist_ld
     mix = cmix;
     cN = act ? row_of(s, piv)[L] : 0;
   }
-  wa.nodes += nodes;
 }
 
-// Intersection of ascending a[0,na) and b[0,nb) appended to out (ascending);
+// ---- bitmap path: outside X as a set of row indices ----------------------
+//
+// Arena: S (p ids) | A (p x W) | XR (q x W rows of the outside-X vertices,
+// in vertex-id order, zero rows kept) | frames. A frame holds P, X (W words),
+// E (WX words) and {pivot, mix}. Row index order is vertex-id order, so the
+// lowest-index tie-break equals the list path's id tie-break.
+
+__device__ __forceinline__ uint64_t bits_fixed(uint32_t p, uint32_t q, uint32_t W) {
+  return align8(4ull * p) + 8ull * p * W + 8ull * q * W;
+}
+
+__device__ __forceinline__ uint32_t bits_fs(uint32_t W, uint32_t WX) { return 8 * (2 * W + WX) + 16; }
+
+__device__ __forceinline__ bool place_bits(Sub &s, uint32_t p, uint32_t q, uint32_t W, uint8_t *lds, uint32_t lds_bytes, uint8_t *hbm,
+                           uint64_t hbm_bytes) {
+  const uint32_t WX = (q + 63) / 64;
+  const uint64_t fixed = bits_fixed(p, q, W);
+  const uint32_t FS = bits_fs(W, WX);
+  const uint32_t D = p;
+  uint8_t *base;
+  uint64_t need;
+  if (fixed + 4ull * FS <= lds_bytes) {
+    base = lds;
+    s.fr_lds = lds + fixed;
+    s.lds_levels = (uint32_t)((lds_bytes - fixed) / FS);
+    s.fr_hbm = hbm;
+    need = D > s.lds_levels ? (uint64_t)(D - s.lds_levels) * FS : 0;
+  } else {
+    base = hbm;
+    s.fr_lds = lds;
+    s.lds_levels = lds_bytes / FS;
+    s.fr_hbm = hbm + fixed;
+    need = fixed + (D > s.lds_levels ? (uint64_t)(D - s.lds_levels) * FS : 0);
+  }
+  if (need > hbm_bytes) return false;
+  s.FS = FS;
+  s.WX = WX;
+  s.S = (uint32_t *)base;
+  s.A = (uint64_t *)(base + align8(4ull * p));
+  s.XR = s.A + (uint64_t)p * W;
+  s.XL = nullptr;
+  s.XA = nullptr;
+  s.XV = nullptr;
+  return true;
+}
+
+__device__ __forceinline__ uint64_t bits_need(uint32_t p, uint32_t q, uint32_t lds_bytes) {
+  const uint32_t W = (p + 63) / 64, WX = (q + 63) / 64;
+  const uint64_t fixed = bits_fixed(p, q, W);
+  const uint32_t FS = bits_fs(W, WX);
+  if (fixed + 4ull * FS <= lds_bytes) {
+    const uint32_t lv = (uint32_t)((lds_bytes - fixed) / FS);
+    return p > lv ? (uint64_t)(p - lv) * FS : 0;
+  }
+  const uint32_t lv = lds_bytes / FS;
+  return fixed + (p > lv ? (uint64_t)(p - lv) * FS : 0);
+}
+
+// Builds A over S and the outside-X rows of the q vertices in `ids`; returns
+// the initial outside-X bitmap word of this lane (rows with a neighbour in S).
+__device__ __forceinline__ uint64_t build_bits(const BkParams &pr, Sub &s, const uint32_t *ids) {
+  const uint32_t L = lane_id(), p = s.p, W = s.W, q = s.q;
+  zero_words(s.A, (uint64_t)p * W);
+  zero_words(s.XR, (uint64_t)q * W);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  for (uint32_t i = 0; i < p; ++i) {
+    const uint32_t u = s.S[i];
+    const uint64_t b = pr.split[u], e = pr.rp[u + 1];
+    for (uint64_t k0 = b; k0 < e; k0 += kWave) {
+      const uint64_t k = k0 + L;
+      if (k < e) {
+        const uint32_t y = pr.oc[k];
+        const uint32_t j = lower_bound_u32(s.S, p, y);
+        if (j < p && s.S[j] == y) {
+          atomicOr((unsigned long long *)&s.A[(uint64_t)i * W + (j >> 6)], 1ull << (j & 63));
+          atomicOr((unsigned long long *)&s.A[(uint64_t)j * W + (i >> 6)], 1ull << (i & 63));
+        }
+      }
+    }
+  }
+  uint64_t E0 = 0;
+  for (uint32_t b = 0, k = 0; b < q; b += kWave, ++k) {
+    const uint32_t i = b + L;
+    bool any = false;
+    if (i < q) {
+      const uint32_t x = ids[i];
+      uint64_t *row = s.XR + (uint64_t)i * W;
+      for (uint64_t t = pr.split[x]; t < pr.rp[x + 1]; ++t) {
+        const uint32_t y = pr.oc[t];
+        const uint32_t j = lower_bound_u32(s.S, p, y);
+        if (j < p && s.S[j] == y) {
+          row[j >> 6] |= 1ull << (j & 63);
+          any = true;
+        }
+      }
+    }
+    const uint64_t m = ballot(any);
+    if (L == k) E0 = m;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+  return E0;
+}
+
+// Pivot over P|X (index order) then the outside rows in E (row order).
+__device__ __forceinline__ void pick_pivot_bits(const Sub &s, uint64_t Pw, uint64_t Xw, uint64_t Ew, uint32_t &piv,
+                                                uint32_t &best) {
+  const uint32_t L = lane_id(), W = s.W;
+  uint32_t bs = 0, bk = kNone;
+  const uint64_t C = Pw | Xw;
+  for (uint32_t k = 0; k < W; ++k) {
+    const uint64_t ck = rdlane64(C, k);
+    if (ck == 0) continue;
+    if ((ck >> L) & 1ull) {
+      const uint32_t u = k * 64 + L;
+      const uint64_t *row = s.A + (uint64_t)u * W;
+      uint32_t sc = 0;
+      for (uint32_t j = 0; j < W; ++j) sc += popc64(rdlane64(Pw, j) & row[j]);
+      if (bk == kNone || sc > bs) {
+        bs = sc;
+        bk = u;
+      }
+    }
+  }
+  for (uint32_t k = 0; k < s.WX; ++k) {
+    const uint64_t ek = rdlane64(Ew, k);
+    if (ek == 0) continue;
+    if ((ek >> L) & 1ull) {
+      const uint32_t e = k * 64 + L;
+      const uint64_t *row = s.XR + (uint64_t)e * W;
+      uint32_t sc = 0;
+      for (uint32_t j = 0; j < W; ++j) sc += popc64(rdlane64(Pw, j) & row[j]);
+      if (bk == kNone || sc > bs) {
+        bs = sc;
+        bk = s.p + e;
+      }
+    }
+  }
+  const uint32_t mx = wave_max(bk == kNone ? 0u : bs + 1u);
+  piv = wave_min((bk != kNone && bs + 1u == mx) ? bk : kNone);
+  best = mx - 1u;
+}
+
+// Outside rows of E adjacent to S[c] (this lane's word of the result).
+__device__ __forceinline__ uint64_t filter_rows(const Sub &s, uint64_t Ew, uint32_t c) {
+  const uint32_t L = lane_id(), W = s.W, cw = c >> 6, cb = c & 63;
+  uint64_t out = 0;
+  for (uint32_t k = 0; k < s.WX; ++k) {
+    const uint64_t ek = rdlane64(Ew, k);
+    if (ek == 0) continue;
+    const bool t = ((ek >> L) & 1ull) && ((s.XR[(uint64_t)(k * 64 + L) * W + cw] >> cb) & 1ull);
+    const uint64_t m = ballot(t);
+    if (L == k) out = m;
+  }
+  return out;
+}
+
+// Does any outside row of E touch S[c]?
+__device__ __forceinline__ bool rows_any(const Sub &s, uint64_t Ew, uint32_t c) {
+  const uint32_t L = lane_id(), W = s.W, cw = c >> 6, cb = c & 63;
+  for (uint32_t k = 0; k < s.WX; ++k) {
+    const uint64_t ek = rdlane64(Ew, k);
+    if (ek == 0) continue;
+    if (ballot(((ek >> L) & 1ull) && ((s.XR[(uint64_t)(k * 64 + L) * W + cw] >> cb) & 1ull))) return true;
+  }
+  return false;
+}
+
+// Publishes the next child of the shallowest frame with a large enough child
+// (bitmap path): the donor's S, A and outside rows plus the child's P, X, E.
+__device__ __forceinline__ bool donate_bits(const BkParams &pr, Sub &s, uint32_t d, uint64_t &cP, uint64_t &cX, uint64_t cE,
+                            uint32_t cpiv, uint64_t cmix, WaveAcc &wa) {
+  const uint32_t L = lane_id(), W = s.W, WX = s.WX;
+  const bool act = L < W, actx = L < WX;
+  if (s.p > pr.slot_pmax || s.q > pr.slot_xcap) return false;
+  for (uint32_t j = 0; j <= d; ++j) {
+    uint64_t Pj = cP, Xj = cX, Ej = cE, mixj = cmix;
+    uint32_t pivj = cpiv;
+    uint8_t *f = nullptr;
+    if (j < d) {
+      f = frame_at(s, j);
+      const uint64_t *fw = (const uint64_t *)f;
+      Pj = act ? fw[L] : 0;
+      Xj = act ? fw[W + L] : 0;
+      Ej = actx ? fw[2 * W + L] : 0;
+      const uint8_t *meta = f + 8 * (2 * W + WX);
+      pivj = uniform(*(const uint32_t *)meta);
+      mixj = uniform64(*(const uint64_t *)(meta + 8));
+    }
+    const uint64_t Nj = act ? row_of(s, pivj)[L] : 0;
+    const uint64_t B = Pj & ~Nj;
+    const uint64_t bm = ballot(B != 0);
+    if (bm == 0) continue;
+    const uint32_t k0 = ctz64(bm);
+    const uint32_t bit = ctz64(rdlane64(B, k0));
+    const uint32_t c = k0 * 64 + bit;
+    const uint64_t Ac = act ? s.A[(uint64_t)c * W + L] : 0;
+    const uint64_t Pn = Pj & Ac, Xn = Xj & Ac;
+    if (wave_sum(popc64(Pn)) < pr.donate_min_p) continue;
+    const uint64_t En = filter_rows(s, Ej, c);
+    unsigned long long pos = 0;
+    if (L == 0) {
+      pos = ring_claim_push(pr);
+      if (pos != kNoSlot) atomicAdd(pr.ctl + kCtlOutstanding, 1ull);
+    }
+    pos = uniform64(pos);
+    if (pos == kNoSlot) return false;
+    uint8_t *slot = pr.ring + (pos % pr.ring_cap) * (uint64_t)pr.slot_bytes;
+    if (L == 0) {
+      uint32_t *h = (uint32_t *)slot;
+      h[0] = s.root;
+      h[1] = s.rsize + j + 1;
+      h[2] = s.p;
+      h[3] = W;
+      h[4] = s.q;
+      h[5] = kTaskBits;
+      *(uint64_t *)(slot + 24) = mixj + splitmix64(s.S[c]);
+    }
+    uint64_t *sp = (uint64_t *)(slot + pr.slot_oP);
+    if (act) {
+      sp[L] = Pn;
+      sp[pr.slot_wmax + L] = Xn;
+    }
+    if (actx) sp[2 * pr.slot_wmax + L] = En;
+    uint32_t *sS = (uint32_t *)(slot + pr.slot_oS);
+    for (uint32_t i = L; i < s.p; i += kWave) sS[i] = s.S[i];
+    uint64_t *sA = (uint64_t *)(slot + pr.slot_oA);
+    const uint64_t nA = (uint64_t)s.p * W;
+    for (uint64_t i = L; i < nA; i += kWave) sA[i] = s.A[i];
+    uint64_t *sXR = (uint64_t *)(slot + pr.slot_oXR);
+    const uint64_t nR = (uint64_t)s.q * W;
+    for (uint64_t i = L; i < nR; i += kWave) sXR[i] = s.XR[i];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (L == 0) __hip_atomic_store(&pr.seq[pos % pr.ring_cap], pos + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t m = 1ull << bit;
+    if (j == d) {
+      if (L == k0) {
+        cP &= ~m;
+        cX |= m;
+      }
+    } else if (L == k0) {
+      ((uint64_t *)f)[L] = Pj & ~m;
+      ((uint64_t *)f)[W + L] = Xj | m;
+    }
+    wa.nodes++;
+    wa.donations++;
+    return true;
+  }
+  return false;
+}
+
+// Runs BK from (P, X, E) on an arena that build_bits (or a received task)
+// has filled.
+template <bool kWL, int kStat>
+__device__ __forceinline__ void solve_bits(const BkParams &pr, Sub &s, uint64_t cP, uint64_t cX, uint64_t cE,
+                           unsigned long long *hist_lds, WaveAcc &wa, Clock<kStat> &clk) {
+  const uint32_t L = lane_id(), W = s.W, WX = s.WX;
+  const bool act = L < W, actx = L < WX;
+  uint32_t piv, best;
+  clk.to(kBdPivot);
+  pick_pivot_bits(s, cP, cX, cE, piv, best);
+  if (piv == kNone || best == wave_sum(popc64(cP))) return;
+  uint64_t cN = act ? row_of(s, piv)[L] : 0;
+  uint64_t mix = s.rmix;
+  uint32_t d = 0;
+  while (true) {
+    if constexpr (kWL) {
+      if (wa.nodes - wa.last_poll >= pr.donate_check) {
+        wa.last_poll = wa.nodes;
+        clk.to(kBdQueue);
+        wa.budget = donation_budget(pr);
+      }
+      if (wa.budget > 0) {
+        clk.to(kBdQueue);
+        if (donate_bits(pr, s, d, cP, cX, cE, piv, mix, wa)) wa.budget--;
+        else wa.budget = 0;
+      }
+    }
+    clk.to(kBdSetOps);
+    const uint64_t B = cP & ~cN;
+    const uint64_t bm = ballot(B != 0);
+    if (bm == 0) {
+      if (d == 0) break;
+      clk.to(kBdBranch);
+      --d;
+      const uint8_t *f = frame_at(s, d);
+      const uint64_t *fw = (const uint64_t *)f;
+      if (act) {
+        cP = fw[L];
+        cX = fw[W + L];
+      }
+      cE = actx ? fw[2 * W + L] : 0;
+      const uint8_t *meta = f + 8 * (2 * W + WX);
+      piv = uniform(*(const uint32_t *)meta);
+      mix = uniform64(*(const uint64_t *)(meta + 8));
+      cN = act ? row_of(s, piv)[L] : 0;
+      continue;
+    }
+    const uint32_t k0 = ctz64(bm);
+    const uint32_t bit = ctz64(rdlane64(B, k0));
+    const uint32_t c = k0 * 64 + bit;
+    if (L == k0) {
+      cP &= ~(1ull << bit);
+      cX |= 1ull << bit;
+    }
+    const uint64_t Ac = act ? s.A[(uint64_t)c * W + L] : 0;
+    const uint64_t Pn = cP & Ac, Xn = cX & Ac;
+    wa.nodes++;
+    const uint64_t cmix = mix + splitmix64(uniform(s.S[c]));
+    if (ballot(Pn != 0) == 0) {
+      clk.to(kBdMaximal);
+      if (ballot(Xn != 0) == 0 && !rows_any(s, cE, c)) report(pr, hist_lds, wa, s.rsize + d + 1, cmix);
+      continue;
+    }
+    const uint64_t En = filter_rows(s, cE, c);
+    uint32_t npiv, nbest;
+    clk.to(kBdPivot);
+    pick_pivot_bits(s, Pn, Xn, En, npiv, nbest);
+    if (npiv == kNone || nbest == wave_sum(popc64(Pn))) continue;
+    clk.to(kBdBranch);
+    uint8_t *f = frame_at(s, d);
+    uint64_t *fw = (uint64_t *)f;
+    if (act) {
+      fw[L] = cP;
+      fw[W + L] = cX;
+    }
+    if (actx) fw[2 * W + L] = cE;
+    uint8_t *meta = f + 8 * (2 * W + WX);
+    *(uint32_t *)meta = piv;
+    *(uint64_t *)(meta + 8) = mix;
+    ++d;
+    cP = Pn;
+    cX = Xn;
+    cE = En;
+    piv = npiv;
+    mix = cmix;
+    cN = act ? row_of(s, piv)[L] : 0;
+  }
+}
+
+// Intersection of ascending a[0,na) and b[0,nb) written to out (ascending);
 // iterates the shorter list and binary-searches the longer.
-__device__ uint32_t intersect_into(const uint32_t *a, uint32_t na, const uint32_t *b, uint32_t nb, uint32_t *out) {
+__device__ __forceinline__ uint32_t intersect_into(const uint32_t *a, uint32_t na, const uint32_t *b, uint32_t nb, uint32_t *out) {
   if (na > nb) {
     const uint32_t *t = a;
     a = b;
@@ -425,13 +1052,12 @@ __device__ uint32_t intersect_into(const uint32_t *a, uint32_t na, const uint32_
   return w;
 }
 
-// Forward copy of n words, one chunk of 64 read before it is written, so a
+// Forward copy of n words; each chunk of 64 is read before it is written, so a
 // destination below an overlapping source is safe.
 __device__ __forceinline__ void wave_copy(uint32_t *dst, const uint32_t *src, uint32_t n) {
   for (uint32_t b = 0; b < n; b += kWave) {
     const uint32_t i = b + lane_id();
     const uint32_t x = i < n ? src[i] : 0;
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     if (i < n) dst[i] = x;
   }
 }
@@ -453,7 +1079,42 @@ __device__ __forceinline__ void defer(const BkParams &pr, uint64_t it) {
   }
 }
 
-template <bool kIPX>
+// All-ones P over the first p bits (lane k holds word k).
+__device__ __forceinline__ uint64_t full_word(uint32_t p) {
+  const uint32_t L = lane_id(), W = (p + 63) / 64;
+  if (L >= W) return 0;
+  return (L == W - 1 && (p & 63)) ? ((1ull << (p & 63)) - 1) : ~0ull;
+}
+
+template <bool kIPX, bool kWL, int kStat>
+__device__ __forceinline__ void finish_sub(const BkParams &pr, Sub &s, uint64_t P0, uint64_t X0, unsigned long long *hist_lds,
+                           WaveAcc &wa, Clock<kStat> &clk) {
+  const uint64_t nodes0 = wa.nodes;
+  clk.to(kBdInduced);
+  build<kIPX>(pr, s);
+  bk_solve<kIPX, kWL, kStat>(pr, s, P0, X0, hist_lds, wa, clk);
+  clk.to(kBdOther);
+  if (lane_id() == 0) {
+    if (pr.sub_nodes) atomicAdd(&pr.sub_nodes[s.root], (unsigned long long)(wa.nodes - nodes0));
+    if (pr.sub_cliques && wa.sub_cliques) atomicAdd(&pr.sub_cliques[s.root], (unsigned long long)wa.sub_cliques);
+  }
+}
+
+template <bool kWL, int kStat>
+__device__ __forceinline__ void finish_bits(const BkParams &pr, Sub &s, const uint32_t *ids,
+                                            unsigned long long *hist_lds, WaveAcc &wa, Clock<kStat> &clk) {
+  const uint64_t nodes0 = wa.nodes;
+  clk.to(kBdInduced);
+  const uint64_t E0 = build_bits(pr, s, ids);
+  solve_bits<kWL, kStat>(pr, s, full_word(s.p), 0, E0, hist_lds, wa, clk);
+  clk.to(kBdOther);
+  if (lane_id() == 0) {
+    if (pr.sub_nodes) atomicAdd(&pr.sub_nodes[s.root], (unsigned long long)(wa.nodes - nodes0));
+    if (pr.sub_cliques && wa.sub_cliques) atomicAdd(&pr.sub_cliques[s.root], (unsigned long long)wa.sub_cliques);
+  }
+}
+
+template <bool kIPX, bool kWL, int kStat>
 __global__ __launch_bounds__(256) void bk_kernel(BkParams pr) {
   extern __shared__ __align__(16) uint8_t smem[];
   unsigned long long *hist_lds = (unsigned long long *)smem;
@@ -464,13 +1125,17 @@ __global__ __launch_bounds__(256) void bk_kernel(BkParams pr) {
   const uint32_t gw = blockIdx.x * pr.wpg + wig;
   uint8_t *hbm = pr.hbm + (uint64_t)gw * pr.hbm_bytes;
   WaveAcc wa;
+  Clock<kStat> clk;
+  clk.start();
   Sub s;
   s.oc = pr.oc;
   s.rp = pr.rp;
   s.split = pr.split;
   const uint32_t *items1 = (const uint32_t *)pr.items;
   const uint2 *items2 = (const uint2 *)pr.items;
+  // phase 1: first-level subproblems by ticket
   while (true) {
+    clk.to(kBdQueue);
     uint32_t t = 0;
     if (L == 0) t = atomicAdd(pr.ticket, 1u);
     t = uniform(t);
@@ -484,18 +1149,18 @@ __global__ __launch_bounds__(256) void bk_kernel(BkParams pr) {
     }
     if (lo >= pr.n_items) break;
     hi = min(hi, pr.n_items);
+    clk.to(kBdInduced);
     for (uint64_t it = lo; it < hi; ++it) {
-      const uint64_t nodes0 = wa.nodes;
       wa.sub_cliques = 0;
-      uint32_t first_v;
       if (pr.level == 1) {
         const uint32_t v = uniform(items1[it]);
-        first_v = v;
         const uint64_t b0 = pr.rp[v], s0 = pr.split[v], e0 = pr.rp[v + 1];
         const uint32_t p = (uint32_t)(e0 - s0), q = (uint32_t)(s0 - b0);
         if (p == 0) continue;
         const uint32_t W = (p + 63) / 64;
-        if (!place(s, p, q, W, kIPX, lds, pr.lds_bytes, hbm, pr.hbm_bytes)) {
+        const bool bits = kIPX && q <= kMaxXRows;
+        if (!(bits ? place_bits(s, p, q, W, lds, pr.lds_bytes, hbm, pr.hbm_bytes)
+                   : place(s, p, q, W, kIPX, lds, pr.lds_bytes, hbm, pr.hbm_bytes))) {
           defer(pr, it);
           continue;
         }
@@ -503,13 +1168,18 @@ __global__ __launch_bounds__(256) void bk_kernel(BkParams pr) {
         s.q = q;
         s.W = W;
         s.rsize = 1;
+        s.root = v;
         s.rmix = splitmix64(v);
         for (uint32_t i = L; i < p; i += kWave) s.S[i] = pr.oc[s0 + i];
-        for (uint32_t i = L; i < q; i += kWave) s.XL[i] = pr.oc[b0 + i];
+        if (bits) {
+          finish_bits<kWL, kStat>(pr, s, pr.oc + b0, hist_lds, wa, clk);
+        } else {
+          for (uint32_t i = L; i < q; i += kWave) s.XL[i] = pr.oc[b0 + i];
+          finish_sub<kIPX, kWL, kStat>(pr, s, full_word(p), 0, hist_lds, wa, clk);
+        }
       } else {
         const uint2 a = items2[it];
         const uint32_t u = uniform(a.x), w = uniform(a.y);
-        first_v = u;
         const uint64_t su = pr.split[u], eu = pr.rp[u + 1], bu = pr.rp[u];
         const uint64_t sw = pr.split[w], ew = pr.rp[w + 1], bw = pr.rp[w];
         const uint32_t pb = (uint32_t)min(eu - su, ew - sw), qb = (uint32_t)min(eu - bu, sw - bw);
@@ -526,41 +1196,187 @@ __global__ __launch_bounds__(256) void bk_kernel(BkParams pr) {
           if (pr.sub_cliques && L == 0 && wa.sub_cliques) atomicAdd(&pr.sub_cliques[u], wa.sub_cliques);
           continue;
         }
-        // re-lay the arena for the actual sizes (never larger than the bounds)
+        // re-lay the arena for the actual sizes
         const uint32_t W = (p + 63) / 64;
-        if (!place(s, p, q, W, kIPX, lds, pr.lds_bytes, hbm, pr.hbm_bytes)) {
+        const bool bits = kIPX && q <= kMaxXRows;
+        if (!(bits ? place_bits(s, p, q, W, lds, pr.lds_bytes, hbm, pr.hbm_bytes)
+                   : place(s, p, q, W, kIPX, lds, pr.lds_bytes, hbm, pr.hbm_bytes))) {
           defer(pr, it);
           continue;
         }
+        // the bitmap layout ends before the staging list t.XL when both share
+        // a base, and lives in a different memory otherwise
         if (s.S != t.S) wave_copy(s.S, t.S, p);
-        if (s.XL != t.XL) wave_copy(s.XL, t.XL, q);
+        if (!bits && s.XL != t.XL) wave_copy(s.XL, t.XL, q);
         s.p = p;
         s.q = q;
         s.W = W;
         s.rsize = 2;
+        s.root = u;
         s.rmix = rmix;
-      }
-      build<kIPX>(pr, s);
-      bk_solve<kIPX>(pr, s, hist_lds, wa);
-      if (L == 0) {
-        if (pr.sub_nodes) atomicAdd(&pr.sub_nodes[first_v], (unsigned long long)(wa.nodes - nodes0));
-        if (pr.sub_cliques && wa.sub_cliques) atomicAdd(&pr.sub_cliques[first_v], (unsigned long long)wa.sub_cliques);
+        if (bits) finish_bits<kWL, kStat>(pr, s, t.XL, hist_lds, wa, clk);
+        else finish_sub<kIPX, kWL, kStat>(pr, s, full_word(p), 0, hist_lds, wa, clk);
       }
     }
+    if (kWL && L == 0) atomicSub(pr.ctl + kCtlOutstanding, 1ull);
   }
+  // phase 2: donated subtrees until no work is outstanding
+  if constexpr (kWL) {
+    bool idle = false;
+    uint32_t backoff = 1;
+    uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    while (true) {
+      clk.to(kBdQueue);
+      unsigned long long pos = 0;
+      if (L == 0) pos = ring_claim_pop(pr);
+      pos = uniform64(pos);
+      if (pos != kNoSlot) {
+        if (idle && L == 0) atomicSub(pr.ctl + kCtlIdle, 1ull);
+        idle = false;
+        backoff = 1;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        const uint8_t *slot = pr.ring + (pos % pr.ring_cap) * (uint64_t)pr.slot_bytes;
+        const uint32_t *h = (const uint32_t *)slot;
+        const uint32_t root = uniform(h[0]), rsize = uniform(h[1]), p = uniform(h[2]), W = uniform(h[3]),
+                       nx = uniform(h[4]), kind = uniform(h[5]);
+        const uint64_t rmix = uniform64(*(const uint64_t *)(slot + 24));
+        if (kIPX && kind == kTaskBits) {
+          const bool okb = p <= pr.slot_pmax && W == (p + 63) / 64 && nx <= pr.slot_xcap && nx <= kMaxXRows &&
+                           place_bits(s, p, nx, W, lds, pr.lds_bytes, hbm, pr.hbm_bytes);
+          uint64_t P0 = 0, X0 = 0, E0 = 0;
+          if (okb) {
+            const uint64_t *sp = (const uint64_t *)(slot + pr.slot_oP);
+            if (L < W) {
+              P0 = sp[L];
+              X0 = sp[pr.slot_wmax + L];
+            }
+            if (L < s.WX) E0 = sp[2 * pr.slot_wmax + L];
+            wave_copy(s.S, (const uint32_t *)(slot + pr.slot_oS), p);
+            const uint64_t *sA = (const uint64_t *)(slot + pr.slot_oA);
+            const uint64_t nA = (uint64_t)p * W;
+            for (uint64_t i = L; i < nA; i += kWave) s.A[i] = sA[i];
+            const uint64_t *sXR = (const uint64_t *)(slot + pr.slot_oXR);
+            const uint64_t nR = (uint64_t)nx * W;
+            for (uint64_t i = L; i < nR; i += kWave) s.XR[i] = sXR[i];
+          }
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+          if (L == 0)
+            __hip_atomic_store(&pr.seq[pos % pr.ring_cap], pos + pr.ring_cap, __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+          if (okb) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+            s.p = p;
+            s.q = nx;
+            s.W = W;
+            s.rsize = rsize;
+            s.root = root;
+            s.rmix = rmix;
+            wa.sub_cliques = 0;
+            const uint64_t nodes0 = wa.nodes;
+            solve_bits<kWL, kStat>(pr, s, P0, X0, E0, hist_lds, wa, clk);
+            clk.to(kBdOther);
+            if (L == 0) {
+              if (pr.sub_nodes) atomicAdd(&pr.sub_nodes[root], (unsigned long long)(wa.nodes - nodes0));
+              if (pr.sub_cliques && wa.sub_cliques)
+                atomicAdd(&pr.sub_cliques[root], (unsigned long long)wa.sub_cliques);
+            }
+          } else if (L == 0) {
+            atomicOr(&pr.acc[kAccError], 2ull);
+          }
+          if (L == 0) atomicSub(pr.ctl + kCtlOutstanding, 1ull);
+          t0 = __builtin_amdgcn_s_memrealtime();
+          continue;
+        }
+        const bool ok = kind == kTaskList && p <= pr.slot_pmax && W == (p + 63) / 64 && nx <= pr.slot_xcap &&
+                        place(s, p, nx, W, kIPX, lds, pr.lds_bytes, hbm, pr.hbm_bytes);
+        uint64_t P0 = 0, X0 = 0;
+        if (ok) {
+          const uint64_t *sp = (const uint64_t *)(slot + pr.slot_oP);
+          if (L < W) {
+            P0 = sp[L];
+            X0 = sp[pr.slot_wmax + L];
+          }
+          wave_copy(s.S, (const uint32_t *)(slot + pr.slot_oS), p);
+          const uint64_t *sA = (const uint64_t *)(slot + pr.slot_oA);
+          const uint64_t nA = (uint64_t)p * W;
+          for (uint64_t i = L; i < nA; i += kWave) s.A[i] = sA[i];
+          const uint32_t *sXI = (const uint32_t *)(slot + pr.slot_oXI);
+          if (kIPX) {
+            wave_copy(s.XV, sXI, nx);
+            const uint64_t *sXR = (const uint64_t *)(slot + pr.slot_oXR);
+            const uint64_t nR = (uint64_t)nx * W;
+            for (uint64_t i = L; i < nR; i += kWave) s.XR[i] = sXR[i];
+            for (uint32_t i = L; i < nx; i += kWave) s.XL[i] = i;
+          } else {
+            wave_copy(s.XL, sXI, nx);
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (L == 0)
+          __hip_atomic_store(&pr.seq[pos % pr.ring_cap], pos + pr.ring_cap, __ATOMIC_RELEASE,
+                             __HIP_MEMORY_SCOPE_AGENT);
+        if (ok) {
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+          s.p = p;
+          s.q = nx;
+          s.W = W;
+          s.xe0 = nx;
+          s.rsize = rsize;
+          s.root = root;
+          s.rmix = rmix;
+          wa.sub_cliques = 0;
+          const uint64_t nodes0 = wa.nodes;
+          bk_solve<kIPX, kWL, kStat>(pr, s, P0, X0, hist_lds, wa, clk);
+          clk.to(kBdOther);
+          if (L == 0) {
+            if (pr.sub_nodes) atomicAdd(&pr.sub_nodes[root], (unsigned long long)(wa.nodes - nodes0));
+            if (pr.sub_cliques && wa.sub_cliques) atomicAdd(&pr.sub_cliques[root], (unsigned long long)wa.sub_cliques);
+          }
+        } else if (L == 0) {
+          atomicOr(&pr.acc[kAccError], 2ull);  // donors only publish what fits
+        }
+        if (L == 0) atomicSub(pr.ctl + kCtlOutstanding, 1ull);
+        t0 = __builtin_amdgcn_s_memrealtime();
+        continue;
+      }
+      if (!idle && L == 0) atomicAdd(pr.ctl + kCtlIdle, 1ull);
+      idle = true;
+      const unsigned long long out = uniform64(L == 0 ? ld_relaxed(pr.ctl + kCtlOutstanding) : 0);
+      if (out == 0) break;
+      if (__builtin_amdgcn_s_memrealtime() - t0 > kWatchdogTicks) {
+        if (L == 0) atomicOr(&pr.acc[kAccError], 1ull);
+        break;
+      }
+      // back off up to ~16 x 2K cycles between polls of the shared ring lines
+      for (uint32_t z = 0; z < backoff; ++z) __builtin_amdgcn_s_sleep(32);
+      backoff = min(backoff * 2, 16u);
+    }
+  }
+  clk.to(kBdOther);
   if (L == 0) {
     atomicAdd(&pr.acc[kAccCount], (unsigned long long)wa.count);
     atomicAdd(&pr.acc[kAccHash], (unsigned long long)wa.hash);
     atomicAdd(&pr.acc[kAccNodes], (unsigned long long)wa.nodes);
+    atomicAdd(&pr.acc[kAccDonations], (unsigned long long)wa.donations);
     atomicMax(&pr.acc[kAccMaxSize], (unsigned long long)wa.max_size);
     if (pr.wave_val) {
-      pr.wave_val[gw] = wa.nodes;
+      pr.wave_val[gw] = kStat == kStatDonor ? wa.donations : wa.nodes;
       pr.wave_cu[gw] = hw_cu_id();
+    }
+    if constexpr (kStat == kStatBd) {
+      if (pr.wave_bd)
+        for (int i = 0; i < 7; ++i) pr.wave_bd[(uint64_t)gw * 7 + i] = clk.acc[i];
     }
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < pr.hist_lds; i += blockDim.x)
     if (hist_lds[i]) atomicAdd(&pr.hist[min(i, pr.hist_len - 1)], hist_lds[i]);
+}
+
+__global__ void ring_init(unsigned long long *seq, uint32_t cap, unsigned long long *ctl,
+                          unsigned long long outstanding) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) seq[i] = i;
+  if (blockIdx.x == 0 && threadIdx.x < kCtlWords) ctl[threadIdx.x] = threadIdx.x == kCtlOutstanding ? outstanding : 0;
 }
 
 // Largest HBM arena any item needs, and how many need more than `cap`.
@@ -579,9 +1395,14 @@ __global__ void need_kernel(const uint64_t *rp, const uint64_t *split, const voi
       p = (uint32_t)min(rp[a.x + 1] - split[a.x], rp[a.y + 1] - split[a.y]);
       q = (uint32_t)min(rp[a.x + 1] - rp[a.x], split[a.y] - rp[a.y]);
     }
-    // level-2 items are re-laid for their actual sizes, which can need up to
-    // four more HBM frames than the bounds
-    const uint64_t need = hbm_need(p, q, ipx, lds_bytes) + (level == 2 ? 4ull * (16 * ((p + 63) / 64) + 16) : 0);
+    // level-2 items stage their intersections in the list layout of the
+    // bounds, then re-lay for the actual sizes (up to four more HBM frames)
+    uint64_t need;
+    if (level == 2)
+      need = max(hbm_need(p, q, ipx, lds_bytes), ipx && q <= kMaxXRows ? bits_need(p, q, lds_bytes) : 0ull) +
+             4ull * (8 * (2 * ((p + 63) / 64) + (q + 63) / 64) + 16);
+    else
+      need = ipx && q <= kMaxXRows ? bits_need(p, q, lds_bytes) : hbm_need(p, q, ipx, lds_bytes);
     mx = max(mx, (unsigned long long)need);
     over += need > cap;
   }
@@ -589,13 +1410,57 @@ __global__ void need_kernel(const uint64_t *rp, const uint64_t *split, const voi
   if (over) atomicAdd(&out[1], over);
 }
 
-template <bool kIPX>
-int launch(const BkParams &pr0, uint32_t groups, uint32_t wpg, size_t lds, hipStream_t s) {
-  auto kern = bk_kernel<kIPX>;
+template <bool kIPX, bool kWL, int kStat>
+int launch_one(const BkParams &pr, uint32_t groups, uint32_t wpg, size_t lds, hipStream_t s) {
+  auto kern = bk_kernel<kIPX, kWL, kStat>;
   MCE_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(kern, dim3(groups), dim3(wpg * kWave), lds, s, pr0);
+  hipLaunchKernelGGL(kern, dim3(groups), dim3(wpg * kWave), lds, s, pr);
   MCE_HIP(hipGetLastError());
   return MCE_OK;
+}
+
+template <bool kIPX, bool kWL, int kStat>
+int occupancy_one(int *nb, uint32_t wpg, size_t lds) {
+  MCE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, bk_kernel<kIPX, kWL, kStat>, wpg * kWave, lds));
+  return MCE_OK;
+}
+
+template <bool kIPX, bool kWL>
+int dispatch_stat(int stat, bool occ, int *nb, const BkParams &pr, uint32_t groups, uint32_t wpg, size_t lds,
+                  hipStream_t s) {
+  switch (stat) {
+    case kStatLb:
+      return occ ? occupancy_one<kIPX, kWL, kStatLb>(nb, wpg, lds)
+                 : launch_one<kIPX, kWL, kStatLb>(pr, groups, wpg, lds, s);
+    case kStatDonor:
+      return occ ? occupancy_one<kIPX, kWL, kStatDonor>(nb, wpg, lds)
+                 : launch_one<kIPX, kWL, kStatDonor>(pr, groups, wpg, lds, s);
+    case kStatBd:
+      return occ ? occupancy_one<kIPX, kWL, kStatBd>(nb, wpg, lds)
+                 : launch_one<kIPX, kWL, kStatBd>(pr, groups, wpg, lds, s);
+    default:
+      return occ ? occupancy_one<kIPX, kWL, kStatNone>(nb, wpg, lds)
+                 : launch_one<kIPX, kWL, kStatNone>(pr, groups, wpg, lds, s);
+  }
+}
+
+// Launches (occ == false) or sizes (occ == true) the kernel variant.
+int dispatch(bool ipx, bool wl, int stat, bool occ, int *nb, const BkParams &pr, uint32_t groups, uint32_t wpg,
+             size_t lds, hipStream_t s) {
+  if (ipx)
+    return wl ? dispatch_stat<true, true>(stat, occ, nb, pr, groups, wpg, lds, s)
+              : dispatch_stat<true, false>(stat, occ, nb, pr, groups, wpg, lds, s);
+  return wl ? dispatch_stat<false, true>(stat, occ, nb, pr, groups, wpg, lds, s)
+            : dispatch_stat<false, false>(stat, occ, nb, pr, groups, wpg, lds, s);
+}
+
+int stat_of(int task) {
+  switch (task) {
+    case MCE_TASK_LB_EVAL: return kStatLb;
+    case MCE_TASK_DONOR_EVAL: return kStatDonor;
+    case MCE_TASK_BD_EVAL: return kStatBd;
+    default: return kStatNone;
+  }
 }
 
 }  // namespace
@@ -613,34 +1478,63 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   MCE_HIP(hipGetDevice(&dev));
   hipDeviceProp_t prop;
   MCE_HIP(hipGetDeviceProperties(&prop, dev));
+  const int stat = stat_of(cfg.task);
   const uint32_t wpg = std::max<uint32_t>(1, std::min<uint32_t>(cfg.waves_per_group ? cfg.waves_per_group : 4, 4));
-  const uint32_t lds_wave = cfg.lds_bytes ? cfg.lds_bytes : 8192;
   const uint32_t hist_lds = std::min<uint32_t>(hist_len, 128);
-  const size_t lds = ((8ull * hist_lds + 7) & ~7ull) + (size_t)wpg * lds_wave;
-  if (lds > 160 * 1024) return set_error(MCE_EINVAL, "LDS per workgroup %zu exceeds 160 KiB", lds);
+  const size_t hist_bytes = (8ull * hist_lds + 7) & ~7ull;
+  const size_t lds_cu = 160 * 1024;
+  uint32_t lds_wave = cfg.lds_bytes;
+  if (!lds_wave) {
+    // fill the CU's LDS at the occupancy the registers allow
+    int nb_regs = 0;
+    BkParams probe{};
+    int rc0 = dispatch(cfg.ipx, cfg.wl, stat, true, &nb_regs, probe, 0, wpg, hist_bytes + wpg * 1024, s);
+    if (rc0) return rc0;
+    nb_regs = std::max(nb_regs, 1);
+    const size_t per_group = lds_cu / (size_t)nb_regs;
+    lds_wave = (uint32_t)std::min<size_t>(48 * 1024, ((per_group - hist_bytes) / wpg) & ~63ull);
+    lds_wave = std::max<uint32_t>(lds_wave, 2048);
+  }
+  const size_t lds = hist_bytes + (size_t)wpg * lds_wave;
+  if (lds > lds_cu) return set_error(MCE_EINVAL, "LDS per workgroup %zu exceeds 160 KiB", lds);
+
+  // every device buffer of this call, released on all paths
+  std::vector<void *> bufs;
+  auto dmalloc = [&](void **p, size_t bytes) -> int {
+    if (hipMalloc(p, std::max<size_t>(bytes, 1)) != hipSuccess)
+      return set_error(MCE_ENOMEM, "device allocation of %zu bytes failed", bytes);
+    bufs.push_back(*p);
+    return MCE_OK;
+  };
+  struct Free {
+    std::vector<void *> &b;
+    ~Free() {
+      for (void *p : b) hipFree(p);
+    }
+  } release{bufs};
+  int rc;
 
   unsigned long long *acc = nullptr;
   uint32_t *ticket = nullptr;
-  MCE_HIP(hipMalloc(&acc, sizeof(unsigned long long) * (kAccN + 2)));
-  MCE_HIP(hipMalloc(&ticket, sizeof(uint32_t)));
+  if ((rc = dmalloc((void **)&acc, sizeof(unsigned long long) * (kAccN + 2)))) return rc;
+  if ((rc = dmalloc((void **)&ticket, sizeof(uint32_t)))) return rc;
   unsigned long long *need = acc + kAccN;
   MCE_HIP(hipMemsetAsync(acc, 0, sizeof(unsigned long long) * (kAccN + 2), s));
   const uint64_t cap = 16ull << 20;
-  need_kernel<<<1024, 256, 0, s>>>(g.rp, g.split, d_items, n_items, (uint32_t)cfg.level, cfg.ipx, lds_wave, cap, need);
+  need_kernel<<<1024, 256, 0, s>>>(g.rp, g.split, d_items, n_items, (uint32_t)cfg.level, cfg.ipx, lds_wave, cap,
+                                   need);
   unsigned long long hneed[2];
   MCE_HIP(hipMemcpyAsync(hneed, need, sizeof(hneed), hipMemcpyDeviceToHost, s));
   MCE_HIP(hipStreamSynchronize(s));
 
   int nb = 0;
-  if (cfg.ipx)
-    MCE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, bk_kernel<true>, wpg * kWave, lds));
-  else
-    MCE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, bk_kernel<false>, wpg * kWave, lds));
+  BkParams pr{};
+  if ((rc = dispatch(cfg.ipx, cfg.wl, stat, true, &nb, pr, 0, wpg, lds, s))) return rc;
   nb = std::max(nb, 1);
   uint32_t groups = cfg.groups ? cfg.groups : (uint32_t)(nb * prop.multiProcessorCount);
+  groups = std::min<uint32_t>(groups, (uint32_t)(nb * prop.multiProcessorCount));  // all waves co-resident
   uint64_t arena = std::min<uint64_t>(hneed[0], cap);
   arena = (arena + 255) & ~255ull;
-  // keep the HBM arenas within a quarter of free memory
   size_t freeb = 0, totalb = 0;
   MCE_HIP(hipMemGetInfo(&freeb, &totalb));
   if (arena && (uint64_t)groups * wpg * arena > freeb / 4)
@@ -649,16 +1543,16 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
 
   uint8_t *hbm = nullptr;
   uint64_t *overflow = nullptr;
-  unsigned long long *wave_val = nullptr;
+  unsigned long long *wave_val = nullptr, *wave_bd = nullptr;
   uint32_t *wave_cu = nullptr;
-  if (arena) MCE_HIP(hipMalloc(&hbm, arena * waves));
-  if (hneed[1]) MCE_HIP(hipMalloc(&overflow, sizeof(uint64_t) * hneed[1]));
+  if (arena && (rc = dmalloc((void **)&hbm, arena * waves))) return rc;
+  if (hneed[1] && (rc = dmalloc((void **)&overflow, sizeof(uint64_t) * hneed[1]))) return rc;
   if (wave_stats) {
-    MCE_HIP(hipMalloc(&wave_val, sizeof(unsigned long long) * waves));
-    MCE_HIP(hipMalloc(&wave_cu, sizeof(uint32_t) * waves));
+    if ((rc = dmalloc((void **)&wave_val, sizeof(unsigned long long) * waves))) return rc;
+    if ((rc = dmalloc((void **)&wave_cu, sizeof(uint32_t) * waves))) return rc;
+    if (stat == kStatBd && (rc = dmalloc((void **)&wave_bd, sizeof(unsigned long long) * 7 * waves))) return rc;
   }
 
-  BkParams pr{};
   pr.rp = g.rp;
   pr.ci = g.ci;
   pr.oc = g.oc;
@@ -682,15 +1576,42 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   pr.overflow_cap = hneed[1];
   pr.wave_val = wave_val;
   pr.wave_cu = wave_cu;
+  pr.wave_bd = wave_bd;
   pr.wpg = wpg;
+  const uint64_t n_tickets = n_items <= pr.t1 ? n_items : pr.t1 + (n_items - pr.t1 + pr.chunk - 1) / pr.chunk;
+  if (n_tickets >= 0xFFFFFF00ull) return set_error(MCE_ELIMIT, "too many work tickets");
+  if (cfg.wl) {
+    // A slot holds a donor's vertex set and bit matrix, the child's P/X words
+    // and its outside-X ids (and rows with IPX). One slot per wave, within a
+    // 2 GiB budget; the outside-X capacity takes what the matrix leaves.
+    auto a8 = [](uint64_t x) { return (x + 7) & ~7ull; };
+    pr.slot_pmax = std::max<uint32_t>(info.max_out_degree, 1);
+    pr.slot_wmax = (pr.slot_pmax + 63) / 64;
+    pr.slot_oP = kSlotHeader;
+    pr.slot_oS = pr.slot_oP + 8 * (2 * pr.slot_wmax + 64);  // P, X, and the 64-word outside-X bitmap
+    pr.slot_oA = pr.slot_oS + (uint32_t)a8(4ull * pr.slot_pmax);
+    const uint64_t fixed = pr.slot_oA + 8ull * pr.slot_pmax * pr.slot_wmax;
+    const uint64_t per_x = 4ull + (cfg.ipx ? 8ull * pr.slot_wmax : 0);
+    const uint64_t budget = std::max<uint64_t>(fixed + 64 * per_x, (2ull << 30) / std::max<uint32_t>(waves, 1));
+    pr.slot_xcap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(info.max_in_degree, 64), (budget - fixed) / per_x);
+    pr.slot_oXI = (uint32_t)fixed;
+    pr.slot_oXR = (uint32_t)(fixed + a8(4ull * pr.slot_xcap));
+    pr.slot_bytes = ((uint64_t)pr.slot_oXR + (cfg.ipx ? 8ull * pr.slot_xcap * pr.slot_wmax : 0) + 255) & ~255ull;
+    pr.ring_cap = waves;
+    pr.donate_min_p = std::max<uint32_t>(cfg.donate_min_p, 1);
+    pr.donate_check = std::max<uint32_t>(cfg.donate_check_nodes, 1);
+    if ((rc = dmalloc((void **)&pr.ctl, sizeof(unsigned long long) * kCtlWords))) return rc;
+    if ((rc = dmalloc((void **)&pr.seq, sizeof(unsigned long long) * pr.ring_cap))) return rc;
+    if ((rc = dmalloc((void **)&pr.ring, pr.slot_bytes * pr.ring_cap))) return rc;
+    ring_init<<<64, 256, 0, s>>>(pr.seq, pr.ring_cap, pr.ctl, n_tickets);
+  }
 
   hipEvent_t e0, e1;
   MCE_HIP(hipEventCreate(&e0));
   MCE_HIP(hipEventCreate(&e1));
   MCE_HIP(hipMemsetAsync(ticket, 0, sizeof(uint32_t), s));
   MCE_HIP(hipEventRecord(e0, s));
-  int rc = cfg.ipx ? launch<true>(pr, groups, wpg, lds, s) : launch<false>(pr, groups, wpg, lds, s);
-  if (rc) return rc;
+  if ((rc = dispatch(cfg.ipx, cfg.wl, stat, false, nullptr, pr, groups, wpg, lds, s))) return rc;
   // items that did not fit: rerun them with arenas sized for the largest
   unsigned long long novf = 0;
   if (hneed[1]) {
@@ -700,9 +1621,10 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   if (novf > hneed[1]) return set_error(MCE_EINTERNAL, "arena overflow list overrun (%llu > %llu)", novf, hneed[1]);
   if (novf) {
     const uint64_t big = (hneed[0] + 255) & ~255ull;
-    uint32_t g2 = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((novf + wpg - 1) / wpg, (freeb / 2) / ((uint64_t)wpg * big)));
+    const uint32_t g2 = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>((novf + wpg - 1) / wpg, (freeb / 2) / ((uint64_t)wpg * big)));
     uint8_t *hbm2 = nullptr;
-    MCE_HIP(hipMalloc(&hbm2, big * g2 * wpg));
+    if ((rc = dmalloc((void **)&hbm2, big * g2 * wpg))) return rc;
     std::vector<uint64_t> idx(novf);
     MCE_HIP(hipMemcpy(idx.data(), overflow, sizeof(uint64_t) * novf, hipMemcpyDeviceToHost));
     std::sort(idx.begin(), idx.end());
@@ -711,7 +1633,7 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
     for (uint64_t i = 0; i < novf; ++i)
       MCE_HIP(hipMemcpy(hitems.data() + i * isz, (const uint8_t *)d_items + idx[i] * isz, isz, hipMemcpyDeviceToHost));
     void *ditems2 = nullptr;
-    MCE_HIP(hipMalloc(&ditems2, isz * novf));
+    if ((rc = dmalloc(&ditems2, isz * novf))) return rc;
     MCE_HIP(hipMemcpy(ditems2, hitems.data(), isz * novf, hipMemcpyHostToDevice));
     BkParams p2 = pr;
     p2.items = ditems2;
@@ -720,16 +1642,14 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
     p2.hbm = hbm2;
     p2.hbm_bytes = big;
     p2.wave_val = nullptr;
+    p2.wave_bd = nullptr;
     p2.overflow = nullptr;
     p2.overflow_cap = 0;
     MCE_HIP(hipMemsetAsync(ticket, 0, sizeof(uint32_t), s));
-    rc = cfg.ipx ? launch<true>(p2, g2, wpg, lds, s) : launch<false>(p2, g2, wpg, lds, s);
-    if (rc) return rc;
+    if ((rc = dispatch(cfg.ipx, false, kStatNone, false, nullptr, p2, g2, wpg, lds, s))) return rc;
     unsigned long long again = 0;
     MCE_HIP(hipMemcpyAsync(&again, &acc[kAccOverflow], sizeof(again), hipMemcpyDeviceToHost, s));
     MCE_HIP(hipStreamSynchronize(s));
-    hipFree(hbm2);
-    hipFree(ditems2);
     if (again != novf) return set_error(MCE_EINTERNAL, "items did not fit the large arena");
   }
   MCE_HIP(hipEventRecord(e1, s));
@@ -738,6 +1658,10 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   MCE_HIP(hipStreamSynchronize(s));
   float ms = 0;
   MCE_HIP(hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (h[kAccError] & 1ull) return set_error(MCE_EINTERNAL, "worker list watchdog expired (termination not reached)");
+  if (h[kAccError] & 2ull) return set_error(MCE_EINTERNAL, "a donated subproblem did not fit its receiver");
   out->count = h[kAccCount];
   out->hash = h[kAccHash];
   out->nodes = h[kAccNodes];
@@ -746,27 +1670,23 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   out->waves = waves;
   out->ms = ms;
   if (wave_stats && wave_stats_cap) {
-    std::vector<unsigned long long> wv(waves);
+    std::vector<unsigned long long> wv(waves), wb(stat == kStatBd ? 7ull * waves : 0);
     std::vector<uint32_t> wc(waves);
     MCE_HIP(hipMemcpy(wv.data(), wave_val, sizeof(unsigned long long) * waves, hipMemcpyDeviceToHost));
     MCE_HIP(hipMemcpy(wc.data(), wave_cu, sizeof(uint32_t) * waves, hipMemcpyDeviceToHost));
+    if (stat == kStatBd)
+      MCE_HIP(hipMemcpy(wb.data(), wave_bd, sizeof(unsigned long long) * 7 * waves, hipMemcpyDeviceToHost));
     const uint32_t k = std::min<uint32_t>(waves, wave_stats_cap);
     for (uint32_t i = 0; i < k; ++i) {
       wave_stats[i] = mce_wave_stat{};
       wave_stats[i].wave = i;
       wave_stats[i].cu = wc[i];
       wave_stats[i].value = wv[i];
+      if (stat == kStatBd)
+        for (int c = 0; c < 7; ++c) wave_stats[i].bd[c] = wb[(uint64_t)i * 7 + c];
     }
     if (wave_stats_len) *wave_stats_len = k;
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  hipFree(acc);
-  hipFree(ticket);
-  hipFree(hbm);
-  hipFree(overflow);
-  hipFree(wave_val);
-  hipFree(wave_cu);
   return MCE_OK;
 }
 
