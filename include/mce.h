@@ -101,6 +101,7 @@ typedef struct mce_opts {
   const uint32_t *order;   /* MCE_SCHED_GIVEN: first-level vertex order      */
   uint64_t order_len;
   const char *model_path;  /* MCE_SCHED_LEARNED: cost model file             */
+  const struct mce_mlp *model; /* MCE_SCHED_LEARNED: in-memory model (wins over model_path) */
 
   /* optional outputs (NULL = not collected) */
   uint64_t *sub_nodes;     /* n entries: BK tree nodes per first-level vertex */
@@ -136,6 +137,37 @@ typedef struct mce_result {
   uint32_t wave_stats_cap;
   uint32_t wave_stats_len;
 } mce_result;
+
+/* ---- learned cost model (no reference counterpart) ----------------------
+ * A 16 -> 64 -> 64 -> 1 ReLU MLP predicting log(1 + BK tree nodes) of each
+ * first-level subproblem from 16 cheap features of its vertex v
+ * (mce_graph_features): log1p of d(v), d+(v), d-(v); core(v)/degeneracy;
+ * log1p core(v); log1p sum_{u in N+(v)} d+(u); log1p sum_{x in N-(v)} d+(x);
+ * log1p e(P) and the density 2e(P)/(p(p-1)) of the subgraph induced by
+ * P = N+(v); the mean inner degree 2e(P)/p; log1p max_{u in N+(v)} d+(u);
+ * the peel position pri(v)/rounds; log1p sum_{u in N+(v)} d(u);
+ * log1p(p) * density; log1p(p * q); and 1. Training runs on the GPU with
+ * this library's FP32 forward/backward/Adam kernels; inference runs as BF16
+ * MFMA GEMMs with FP32 accumulation. The model only orders work; results never
+ * depend on it. */
+#define MCE_MLP_FEATURES 16
+#define MCE_MLP_HIDDEN 64
+#define MCE_MLP_PARAMS (16 * 64 + 64 + 64 * 64 + 64 + 64 + 1)
+
+typedef struct mce_mlp {
+  float mu[MCE_MLP_FEATURES];    /* feature normalisation: x' = (x - mu) / sigma */
+  float sigma[MCE_MLP_FEATURES];
+  float params[MCE_MLP_PARAMS];  /* W1[16][64] | b1[64] | W2[64][64] | b2[64] | w3[64] | b3 */
+} mce_mlp;
+
+typedef struct mce_train_opts {
+  int device;
+  int epochs;
+  int batch;          /* rows per optimiser step */
+  float lr;           /* Adam step size */
+  float weight_decay; /* L2 on weights, added to the gradient */
+  uint64_t seed;      /* row shuffling */
+} mce_train_opts;
 
 /* ---- library ---------------------------------------------------------- */
 int mce_abi_version(void);
@@ -200,6 +232,23 @@ int mce_graph_count(mce_graph *g, const mce_opts *opts, mce_result *res);
 int mce_graph_download(const mce_graph *g, uint32_t *priority,
                        uint64_t *split, uint32_t *col_ind);
 void mce_graph_destroy(mce_graph *g);
+
+/* Cost-model features of every vertex (rows of MCE_MLP_FEATURES floats) of an
+ * ordered graph. */
+int mce_graph_features(mce_graph *g, float *features);
+
+/* Feature statistics from data and seeded random weights. */
+int mce_mlp_init(mce_mlp *m, const float *x, uint64_t rows, uint64_t seed);
+/* Trains on the GPU (mean squared error on y). loss_out: final epoch loss. */
+int mce_mlp_train(mce_mlp *m, const float *x, const float *y, uint64_t rows, const mce_train_opts *o,
+                  double *loss_out);
+/* Full-batch gradient of the mean squared error (FP32, GPU). */
+int mce_mlp_grad(const mce_mlp *m, const float *x, const float *y, uint64_t rows, int device, float *grad,
+                 double *loss_out);
+/* Predictions for `rows` feature rows (BF16 MFMA, FP32 accumulation). */
+int mce_mlp_predict(const mce_mlp *m, const float *x, uint64_t rows, int device, float *out);
+int mce_mlp_save(const char *path, const mce_mlp *m);
+int mce_mlp_load(const char *path, mce_mlp *m);
 
 /* One call: create + order + count + destroy (src/main.cu:18-187). */
 int mce_run(const mce_csr *csr, const mce_opts *opts, mce_result *res);

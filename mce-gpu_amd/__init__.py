@@ -48,13 +48,27 @@ class Batch(ctypes.Structure):
     _fields_ = [("num_graphs", c_uint32), ("vtx_off", POINTER(c_uint32)), ("csr", Csr)]
 
 
+MLP_FEATURES, MLP_HIDDEN = 16, 64
+MLP_PARAMS = 16 * 64 + 64 + 64 * 64 + 64 + 64 + 1
+
+
+class Mlp(ctypes.Structure):
+    _fields_ = [("mu", ctypes.c_float * MLP_FEATURES), ("sigma", ctypes.c_float * MLP_FEATURES),
+                ("params", ctypes.c_float * MLP_PARAMS)]
+
+
+class TrainOpts(ctypes.Structure):
+    _fields_ = [("device", c_int), ("epochs", c_int), ("batch", c_int), ("lr", ctypes.c_float),
+                ("weight_decay", ctypes.c_float), ("seed", c_uint64)]
+
+
 class Opts(ctypes.Structure):
     _fields_ = [
         ("level", c_int), ("induced", c_int), ("worklist", c_int), ("task", c_int), ("device", c_int),
         ("waves_per_group", c_int), ("groups_per_cu", c_int), ("lds_arena_bytes", c_int),
         ("hbm_arena_bytes", c_int), ("stack_lds_levels", c_int), ("donate_min_p", c_int),
         ("donate_check_nodes", c_int), ("schedule", c_int), ("order", POINTER(c_uint32)),
-        ("order_len", c_uint64), ("model_path", c_char_p), ("sub_nodes", POINTER(c_uint64)),
+        ("order_len", c_uint64), ("model_path", c_char_p), ("model", POINTER(Mlp)), ("sub_nodes", POINTER(c_uint64)),
         ("sub_cliques", POINTER(c_uint64)),
     ]
 
@@ -98,6 +112,15 @@ SIGNATURES = {
     "mce_graph_count": (c_int, [c_void_p, POINTER(Opts), POINTER(Result)]),
     "mce_graph_download": (c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint64), POINTER(c_uint32)]),
     "mce_graph_destroy": (None, [c_void_p]),
+    "mce_graph_features": (c_int, [c_void_p, POINTER(ctypes.c_float)]),
+    "mce_mlp_init": (c_int, [POINTER(Mlp), POINTER(ctypes.c_float), c_uint64, c_uint64]),
+    "mce_mlp_train": (c_int, [POINTER(Mlp), POINTER(ctypes.c_float), POINTER(ctypes.c_float), c_uint64,
+                              POINTER(TrainOpts), POINTER(c_double)]),
+    "mce_mlp_grad": (c_int, [POINTER(Mlp), POINTER(ctypes.c_float), POINTER(ctypes.c_float), c_uint64, c_int,
+                             POINTER(ctypes.c_float), POINTER(c_double)]),
+    "mce_mlp_predict": (c_int, [POINTER(Mlp), POINTER(ctypes.c_float), c_uint64, c_int, POINTER(ctypes.c_float)]),
+    "mce_mlp_save": (c_int, [c_char_p, POINTER(Mlp)]),
+    "mce_mlp_load": (c_int, [c_char_p, POINTER(Mlp)]),
     "mce_run": (c_int, [POINTER(Csr), POINTER(Opts), POINTER(Result)]),
     "mce_run_batch": (c_int, [POINTER(Batch), POINTER(Opts), POINTER(c_uint64), POINTER(Result)]),
 }
@@ -128,6 +151,10 @@ def _u64p(a: np.ndarray):
 
 def _u32p(a: np.ndarray):
     return a.ctypes.data_as(POINTER(c_uint32))
+
+
+def _f32p(a: np.ndarray):
+    return a.ctypes.data_as(POINTER(ctypes.c_float))
 
 
 class Graph:
@@ -261,18 +288,37 @@ class DeviceGraph:
                                t_order_ms=r.t_order_ms, t_orient_ms=r.t_orient_ms)
         return self.order_info
 
-    def download(self):
+    def download(self, prio=True, split=True, col=True):
+        """Ordering results: (priority[n], split[n], oriented col_ind[m]); a
+        part not asked for is returned as None and not copied."""
         n, m = self._host.n, self._host.m
-        prio = np.zeros(max(n, 1), np.uint32)
-        split = np.zeros(max(n, 1), np.uint64)
-        oc = np.zeros(max(m, 1), np.uint32)
-        _check(lib.mce_graph_download(self.handle, _u32p(prio), _u64p(split), _u32p(oc)))
-        return prio[:n], split[:n], oc[:m]
+        P = np.zeros(max(n, 1), np.uint32) if prio else None
+        S = np.zeros(max(n, 1), np.uint64) if split else None
+        C = np.zeros(max(m, 1), np.uint32) if col else None
+        null32, null64 = ctypes.cast(0, POINTER(c_uint32)), ctypes.cast(0, POINTER(c_uint64))
+        _check(lib.mce_graph_download(self.handle, _u32p(P) if prio else null32, _u64p(S) if split else null64,
+                                      _u32p(C) if col else null32))
+        return (P[:n] if prio else None), (S[:n] if split else None), (C[:m] if col else None)
+
+    def features(self) -> np.ndarray:
+        """Cost-model features of every vertex (n x 16 float32); needs order()."""
+        n = self._host.n
+        F = np.zeros((max(n, 1), MLP_FEATURES), np.float32)
+        _check(lib.mce_graph_features(self.handle, _f32p(F)))
+        return F[:n]
 
     def count(self, level=L1, induced=IPX, worklist=NOWL, task=TASK_MCE, order=None, sub_nodes=False,
-              sub_cliques=False, wave_stats_cap=0, **block) -> dict:
+              sub_cliques=False, wave_stats_cap=0, model=None, model_path=None, **block) -> dict:
         o = default_opts(level=level, induced=induced, worklist=worklist, task=task, device=self.device, **block)
         keep = []
+        if model is not None or model_path is not None:
+            o.schedule = SCHED_LEARNED
+            if model is not None:
+                keep.append(model)
+                o.model = ctypes.pointer(model)
+            if model_path is not None:
+                keep.append(model_path.encode())
+                o.model_path = keep[-1]
         if order is not None:
             arr = np.ascontiguousarray(order, dtype=np.uint32)
             keep.append(arr)
@@ -343,3 +389,56 @@ def run(g: Graph, device: int = 0, **kw) -> dict:
         res["t_order_ms"] = info["t_order_ms"]
         res["t_orient_ms"] = info["t_orient_ms"]
         return res
+
+
+# ---- learned cost model -----------------------------------------------------
+
+def mlp_init(x: np.ndarray, seed: int = 1) -> Mlp:
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    m = Mlp()
+    _check(lib.mce_mlp_init(byref(m), _f32p(x), x.shape[0], seed))
+    return m
+
+
+def mlp_params(m: Mlp) -> dict:
+    p = np.ctypeslib.as_array(m.params).copy()
+    o = [0, 16 * 64, 16 * 64 + 64, 16 * 64 + 64 + 64 * 64, 16 * 64 + 64 + 64 * 64 + 64, MLP_PARAMS - 1]
+    return dict(W1=p[o[0]:o[1]].reshape(16, 64), b1=p[o[1]:o[2]], W2=p[o[2]:o[3]].reshape(64, 64), b2=p[o[3]:o[4]],
+                w3=p[o[4]:o[5]], b3=float(p[o[5]]), mu=np.ctypeslib.as_array(m.mu).copy(),
+                sigma=np.ctypeslib.as_array(m.sigma).copy())
+
+
+def mlp_train(m: Mlp, x: np.ndarray, y: np.ndarray, epochs=20, batch=8192, lr=1e-3, weight_decay=0.0, seed=1,
+              device=0) -> float:
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    o = TrainOpts(device, epochs, batch, lr, weight_decay, seed)
+    loss = c_double()
+    _check(lib.mce_mlp_train(byref(m), _f32p(x), _f32p(y), x.shape[0], byref(o), byref(loss)))
+    return loss.value
+
+
+def mlp_grad(m: Mlp, x: np.ndarray, y: np.ndarray, device=0):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    g = np.zeros(MLP_PARAMS, np.float32)
+    loss = c_double()
+    _check(lib.mce_mlp_grad(byref(m), _f32p(x), _f32p(y), x.shape[0], device, _f32p(g), byref(loss)))
+    return g, loss.value
+
+
+def mlp_predict(m: Mlp, x: np.ndarray, device=0) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.zeros(max(x.shape[0], 1), np.float32)
+    _check(lib.mce_mlp_predict(byref(m), _f32p(x), x.shape[0], device, _f32p(out)))
+    return out[:x.shape[0]]
+
+
+def mlp_save(path: str, m: Mlp) -> None:
+    _check(lib.mce_mlp_save(path.encode(), byref(m)))
+
+
+def mlp_load(path: str) -> Mlp:
+    m = Mlp()
+    _check(lib.mce_mlp_load(path.encode(), byref(m)))
+    return m

@@ -142,7 +142,8 @@ int mce_graph_create(const mce_csr *csr, int device, mce_graph **out) {
       hipMalloc(&g.oc, sizeof(uint32_t) * m1) != hipSuccess ||
       hipMalloc(&g.split, sizeof(uint64_t) * std::max<size_t>(g.n, 1)) != hipSuccess ||
       hipMalloc(&g.prio, sizeof(uint32_t) * std::max<size_t>(g.n, 1)) != hipSuccess ||
-      hipMalloc(&g.dplus, sizeof(uint32_t) * std::max<size_t>(g.n, 1)) != hipSuccess)
+      hipMalloc(&g.dplus, sizeof(uint32_t) * std::max<size_t>(g.n, 1)) != hipSuccess ||
+      hipMalloc(&g.core, sizeof(uint32_t) * std::max<size_t>(g.n, 1)) != hipSuccess)
     return fail(set_error(MCE_ENOMEM, "device allocation for the graph (n=%u, m=%llu) failed", g.n,
                           (unsigned long long)g.m));
   if (g.n) {
@@ -182,6 +183,7 @@ void mce_graph_destroy(mce_graph *h) {
   hipFree(h->g.split);
   hipFree(h->g.prio);
   hipFree(h->g.dplus);
+  hipFree(h->g.core);
   if (h->s) hipStreamDestroy(h->s);
   delete h;
 }
@@ -242,6 +244,7 @@ int mce_graph_count(mce_graph *h, const mce_opts *opts, mce_result *res) {
   hipStream_t s = h->s;
   fill_order_fields(h, res);
   res->count = res->hash = res->tree_nodes = res->donations = res->fallback_events = 0;
+  uint64_t fallback = 0;
   res->max_clique_size = 0;
   res->wave_stats_len = 0;
   res->t_count_ms = 0;
@@ -258,6 +261,7 @@ int mce_graph_count(mce_graph *h, const mce_opts *opts, mce_result *res) {
     hipFree(arcs);
   };
   uint32_t len = 0;
+  bool heavy_tail = false;
   if (g.n) {
     if (hipMalloc(&order, sizeof(uint32_t) * g.n) != hipSuccess) {
       cleanup();
@@ -279,11 +283,40 @@ int mce_graph_count(mce_graph *h, const mce_opts *opts, mce_result *res) {
       len = (uint32_t)keep.size();
       if (len) MCE_HIP(hipMemcpy(order, keep.data(), sizeof(uint32_t) * len, hipMemcpyHostToDevice));
     } else {
-      // plain: heaviest-first by out-degree
-      rc = schedule_by_key(g, g.dplus, std::max<uint32_t>(h->info.max_out_degree, 1), order, &len, s);
-      if (rc) {
-        cleanup();
-        return rc;
+      bool learned = false;
+      if (opts->schedule == MCE_SCHED_LEARNED) {
+        // heaviest-first by the cost model; any failure falls back to plain
+        mce_mlp model;
+        bool have = false;
+        if (opts->model) {
+          model = *opts->model;
+          have = mlp_valid(&model);
+        } else if (opts->model_path) {
+          have = mce_mlp_load(opts->model_path, &model) == MCE_OK;
+        }
+        uint32_t *keys = nullptr;
+        if (have && hipMalloc(&keys, sizeof(uint32_t) * g.n) == hipSuccess) {
+          uint32_t bad = 0;
+          double wsum = 0, wmax = 0;
+          rc = learned_keys(g, h->info, model, keys, &bad, &wsum, &wmax, s);
+          if (rc == MCE_OK && bad == 0) {
+            rc = schedule_by_key(g, keys, 4095, order, &len, s);
+            learned = rc == MCE_OK;
+            // a predicted subtree larger than a wave's share: donate early and small
+            if (learned && wmax * 4096.0 > wsum) heavy_tail = true;
+          }
+          hipFree(keys);
+        }
+        if (!learned) fallback++;
+        clear_error();
+      }
+      if (!learned) {
+        // plain: heaviest-first by out-degree
+        rc = schedule_by_key(g, g.dplus, std::max<uint32_t>(h->info.max_out_degree, 1), order, &len, s);
+        if (rc) {
+          cleanup();
+          return rc;
+        }
       }
     }
   }
@@ -319,6 +352,11 @@ int mce_graph_count(mce_graph *h, const mce_opts *opts, mce_result *res) {
   cfg.lds_bytes = (uint32_t)opts->lds_arena_bytes;
   cfg.donate_min_p = (uint32_t)opts->donate_min_p;
   cfg.donate_check_nodes = opts->donate_check_nodes ? (uint32_t)opts->donate_check_nodes : 64;
+  if (heavy_tail && !opts->donate_check_nodes) {
+    // thresholds set from the cost model: a heavy tail wants early, fine donation
+    cfg.donate_check_nodes = 16;
+    cfg.donate_min_p = std::min<uint32_t>(cfg.donate_min_p, 6);
+  }
   if (opts->groups_per_cu) {
     int dev = 0;
     hipDeviceProp_t prop;
@@ -346,6 +384,7 @@ int mce_graph_count(mce_graph *h, const mce_opts *opts, mce_result *res) {
   if (opts->sub_nodes && g.n) MCE_HIP(hipMemcpy(opts->sub_nodes, sub_nodes, sizeof(uint64_t) * g.n, hipMemcpyDeviceToHost));
   if (opts->sub_cliques && g.n)
     MCE_HIP(hipMemcpy(opts->sub_cliques, sub_cliques, sizeof(uint64_t) * g.n, hipMemcpyDeviceToHost));
+  res->fallback_events = fallback;
   res->count = out.count;
   res->hash = out.hash;
   res->tree_nodes = out.nodes;
@@ -354,6 +393,25 @@ int mce_graph_count(mce_graph *h, const mce_opts *opts, mce_result *res) {
   res->num_waves = out.waves;
   res->t_count_ms = out.ms;
   cleanup();
+  return MCE_OK;
+}
+
+int mce_graph_features(mce_graph *h, float *features) {
+  clear_error();
+  if (!h || (!features && h->g.n)) return set_error(MCE_EINVAL, "null argument");
+  if (!h->ordered) return set_error(MCE_EINVAL, "graph is not ordered (call mce_graph_order first)");
+  if (h->g.n == 0) return MCE_OK;
+  MCE_HIP(hipSetDevice(h->device));
+  float *F = nullptr;
+  if (hipMalloc(&F, sizeof(float) * (uint64_t)h->g.n * MCE_MLP_FEATURES) != hipSuccess)
+    return set_error(MCE_ENOMEM, "feature buffer allocation failed");
+  int rc = features_device(h->g, h->info, F, h->s);
+  hipError_t e = hipSuccess;
+  if (!rc) e = hipMemcpyAsync(features, F, sizeof(float) * (uint64_t)h->g.n * MCE_MLP_FEATURES, hipMemcpyDeviceToHost, h->s);
+  if (!rc && e == hipSuccess) e = hipStreamSynchronize(h->s);
+  hipFree(F);
+  if (rc) return rc;
+  if (e != hipSuccess) return hip_fail(e, "features");
   return MCE_OK;
 }
 

@@ -20,10 +20,12 @@ struct DevGraph {
   uint64_t *split = nullptr;
   uint32_t *prio = nullptr;
   uint32_t *dplus = nullptr;  // out-degree after orientation
+  uint32_t *core = nullptr;   // core number (level at which the peel removed v)
 };
 
 struct OrderInfo {
   uint32_t degeneracy = 0;
+  uint32_t n_rounds = 0;
   uint32_t rounds = 0;
   uint32_t max_degree = 0;
   uint32_t max_out_degree = 0;
@@ -74,6 +76,15 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
            uint32_t *wave_stats_len, hipStream_t s, BkOut *out);
 
 int hip_fail(hipError_t e, const char *what);
+
+// Cost model (mlp.hip).
+int features_device(const DevGraph &g, const OrderInfo &info, float *F, hipStream_t s);
+int predict_device(const mce_mlp &m, const float *F, uint64_t rows, float *out, hipStream_t s);
+bool mlp_valid(const mce_mlp *m);
+// Schedule keys from the model's predictions; `bad` counts non-finite ones;
+// work_sum / work_max are the predicted tree sizes (expm1 of the predictions).
+int learned_keys(const DevGraph &g, const OrderInfo &info, const mce_mlp &m, uint32_t *keys, uint32_t *bad,
+                 double *work_sum, double *work_max, hipStream_t s);
 
 }  // namespace mce
 

@@ -61,8 +61,9 @@ __global__ void kc_init(uint32_t n, const uint64_t *__restrict__ rp, int32_t *__
 
 // Level step: compacts the alive list and selects the frontier at level+1.
 __global__ __launch_bounds__(kBlock) void kc_level(PeelState *st, const int32_t *__restrict__ deg,
-                                                   uint32_t *__restrict__ prio, uint32_t *alive0,
-                                                   uint32_t *alive1, uint32_t *front0, uint32_t *front1) {
+                                                   uint32_t *__restrict__ prio, uint32_t *__restrict__ core,
+                                                   uint32_t *alive0, uint32_t *alive1, uint32_t *front0,
+                                                   uint32_t *front1) {
   if (st->mode != kModeLevel) return;
   const int32_t L = st->level + 1;
   const uint32_t ac = st->alive_cur, cnt = st->acnt, round = st->round;
@@ -84,6 +85,7 @@ __global__ __launch_bounds__(kBlock) void kc_level(PeelState *st, const int32_t 
     const uint32_t fs = wave_append(&st->fcnt, sel);
     if (sel) {
       prio[v] = round;
+      core[v] = (uint32_t)L;
       F[fs] = v;
     }
     const uint32_t as = wave_append(&st->acnt2, alive && !sel);
@@ -93,8 +95,8 @@ __global__ __launch_bounds__(kBlock) void kc_level(PeelState *st, const int32_t 
 
 // Peel round: one wave per frontier vertex decrements its alive neighbours.
 __global__ __launch_bounds__(kBlock) void kc_peel(PeelState *st, const uint64_t *__restrict__ rp,
-                                                  const uint32_t *__restrict__ ci, int32_t *deg,
-                                                  uint32_t *prio, uint32_t *front0, uint32_t *front1) {
+                                                  const uint32_t *__restrict__ ci, int32_t *deg, uint32_t *prio,
+                                                  uint32_t *__restrict__ core, uint32_t *front0, uint32_t *front1) {
   if (st->mode != kModePeel) return;
   const int32_t L = st->level;
   const uint32_t cnt = st->fcnt, nround = st->round + 1;
@@ -115,7 +117,8 @@ __global__ __launch_bounds__(kBlock) void kc_peel(PeelState *st, const uint64_t 
           const int32_t old = atomicSub(&deg[w], 1);
           if (old == L + 1) {
             add = true;
-            prio[w] = nround;
+            __hip_atomic_store(&prio[w], nround, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            core[w] = (uint32_t)L;
           }
         }
       }
@@ -345,8 +348,8 @@ int peel(const DevGraph &g, hipStream_t s, OrderInfo *info) {
   uint32_t batch = 8;
   for (uint64_t it = 0;; it += batch) {
     for (uint32_t b = 0; b < batch; ++b) {
-      kc_level<<<lgrid, kBlock, 0, s>>>(st, deg, g.prio, alive, alive + n, front, front + n);
-      kc_peel<<<pgrid, kBlock, 0, s>>>(st, g.rp, g.ci, deg, g.prio, front, front + n);
+      kc_level<<<lgrid, kBlock, 0, s>>>(st, deg, g.prio, g.core, alive, alive + n, front, front + n);
+      kc_peel<<<pgrid, kBlock, 0, s>>>(st, g.rp, g.ci, deg, g.prio, g.core, front, front + n);
       kc_step<<<1, 1, 0, s>>>(st);
     }
     hipError_t e = hipMemcpyAsync(hst, st, sizeof(PeelState), hipMemcpyDeviceToHost, s);
@@ -364,6 +367,7 @@ int peel(const DevGraph &g, hipStream_t s, OrderInfo *info) {
   }
   info->degeneracy = hst->degeneracy;
   info->rounds = hst->round;
+  info->n_rounds = hst->round;
   cleanup();
   return rc;
 }
