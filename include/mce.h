@@ -254,9 +254,19 @@ int mce_mlp_load(const char *path, mce_mlp *m);
 int mce_run(const mce_csr *csr, const mce_opts *opts, mce_result *res);
 
 /* Batched engine: all graphs of a packed batch in one persistent,
- * graph-captured launch; counts_out[g] receives graph g's clique count. */
+ * graph-captured launch; counts_out[g] receives graph g's clique count.
+ * No reference counterpart (the reference runs one graph per process). */
 int mce_run_batch(const mce_batch *batch, const mce_opts *opts,
                   uint64_t *counts_out, mce_result *agg);
+
+/* Autotuner: times candidate block settings (LDS arena per wave, waves per
+ * workgroup, BK frames kept in LDS, donation thresholds) on a stride sample
+ * of the first-level subproblems of an ordered graph and writes the fastest
+ * into *best; report (nullable) receives a JSON list of every trial. Results
+ * do not depend on the settings; a trial that changes them is an error.
+ * No reference counterpart (its block size is a fixed rule, mce.cuh:125-126). */
+int mce_autotune(mce_graph *g, const mce_opts *base, double sample_fraction,
+                 mce_opts *best, char *report, uint64_t report_len);
 
 #ifdef __cplusplus
 }

@@ -123,6 +123,7 @@ SIGNATURES = {
     "mce_mlp_load": (c_int, [c_char_p, POINTER(Mlp)]),
     "mce_run": (c_int, [POINTER(Csr), POINTER(Opts), POINTER(Result)]),
     "mce_run_batch": (c_int, [POINTER(Batch), POINTER(Opts), POINTER(c_uint64), POINTER(Result)]),
+    "mce_autotune": (c_int, [c_void_p, POINTER(Opts), c_double, POINTER(Opts), ctypes.c_char_p, c_uint64]),
 }
 
 
@@ -352,6 +353,16 @@ class DeviceGraph:
             out["wave_stats"] = [(ws[i].wave, ws[i].cu, ws[i].value, tuple(ws[i].bd)) for i in range(r.wave_stats_len)]
         return out
 
+    def autotune(self, sample_fraction=0.05, level=L1, induced=IPX, worklist=WL, **block):
+        """Fastest block settings on a stride sample; returns (settings, trials)."""
+        import json as _json
+        base = default_opts(level=level, induced=induced, worklist=worklist, device=self.device, **block)
+        best = Opts()
+        buf = ctypes.create_string_buffer(1 << 16)
+        _check(lib.mce_autotune(self.handle, byref(base), sample_fraction, byref(best), buf, len(buf)))
+        keys = ("lds_arena_bytes", "waves_per_group", "stack_lds_levels", "donate_min_p", "donate_check_nodes")
+        return {k: getattr(best, k) for k in keys}, _json.loads(buf.value.decode() or "[]")
+
     def close(self) -> None:
         if self.handle:
             lib.mce_graph_destroy(self.handle)
@@ -379,6 +390,23 @@ def _result_dict(r: Result, hist: np.ndarray) -> dict:
         donations=int(r.donations), fallback_events=int(r.fallback_events), t_order_ms=float(r.t_order_ms),
         t_orient_ms=float(r.t_orient_ms), t_count_ms=float(r.t_count_ms),
     )
+
+
+def run_batch(off: np.ndarray, g: Graph, device: int = 0, **kw):
+    """All graphs of a packed batch in one launch; returns (per-graph counts, result)."""
+    off = np.ascontiguousarray(off, dtype=np.uint32)
+    b = Batch()
+    b.num_graphs = off.size - 1
+    b.vtx_off = _u32p(off)
+    b.csr = g.csr()
+    o = default_opts(device=device, **kw)
+    counts = np.zeros(max(off.size - 1, 1), np.uint64)
+    r = Result()
+    hist = np.zeros(HIST_LEN, np.uint64)
+    r.hist = _u64p(hist)
+    r.hist_len = HIST_LEN
+    _check(lib.mce_run_batch(byref(b), byref(o), _u64p(counts), byref(r)))
+    return counts[:off.size - 1], _result_dict(r, hist)
 
 
 def run(g: Graph, device: int = 0, **kw) -> dict:

@@ -9,6 +9,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <string>
 #include <vector>
 
 #include "engine.hpp"
@@ -231,8 +232,14 @@ int mce_graph_download(const mce_graph *h, uint32_t *priority, uint64_t *split, 
   return MCE_OK;
 }
 
+static int count_impl(mce_graph *h, const mce_opts *opts, mce_result *res, bool capture);
+
 int mce_graph_count(mce_graph *h, const mce_opts *opts, mce_result *res) {
   clear_error();
+  return count_impl(h, opts, res, false);
+}
+
+static int count_impl(mce_graph *h, const mce_opts *opts, mce_result *res, bool capture) {
   if (!h || !opts || !res) return set_error(MCE_EINVAL, "null argument");
   int rc = check_opts(opts);
   if (rc) return rc;
@@ -350,8 +357,10 @@ int mce_graph_count(mce_graph *h, const mce_opts *opts, mce_result *res) {
   cfg.task = opts->task;
   cfg.waves_per_group = opts->waves_per_group ? (uint32_t)opts->waves_per_group : 4;
   cfg.lds_bytes = (uint32_t)opts->lds_arena_bytes;
+  cfg.stack_lds_levels = (uint32_t)opts->stack_lds_levels;
   cfg.donate_min_p = (uint32_t)opts->donate_min_p;
   cfg.donate_check_nodes = opts->donate_check_nodes ? (uint32_t)opts->donate_check_nodes : 64;
+  cfg.capture = capture;
   if (heavy_tail && !opts->donate_check_nodes) {
     // thresholds set from the cost model: a heavy tail wants early, fine donation
     cfg.donate_check_nodes = 16;
@@ -437,7 +446,150 @@ int mce_run(const mce_csr *csr, const mce_opts *opts, mce_result *res) {
 int mce_run_batch(const mce_batch *batch, const mce_opts *opts, uint64_t *counts_out, mce_result *agg) {
   clear_error();
   if (!batch || !opts || !counts_out || !agg) return set_error(MCE_EINVAL, "null argument");
-  return set_error(MCE_EINVAL, "batched engine is not available in this build");
+  int rc = check_opts(opts);
+  if (rc) return rc;
+  if (opts->schedule == MCE_SCHED_GIVEN) return set_error(MCE_EINVAL, "a batch takes the plain or learned schedule");
+  const mce_csr &c = batch->csr;
+  const uint32_t G = batch->num_graphs;
+  if (!batch->vtx_off || batch->vtx_off[0] != 0 || batch->vtx_off[G] != c.n)
+    return set_error(MCE_EINVAL, "vtx_off must start at 0 and end at n");
+  if (c.n && (c.row_ptr[0] != 0 || c.row_ptr[c.n] != c.m)) return set_error(MCE_EGRAPH, "bad row_ptr");
+  // arcs stay inside their graph: rows are ascending, so first and last suffice
+  for (uint32_t gi = 0; gi < G; ++gi) {
+    const uint32_t a = batch->vtx_off[gi], b = batch->vtx_off[gi + 1];
+    if (b < a) return set_error(MCE_EINVAL, "vtx_off decreases at graph %u", gi);
+    for (uint32_t v = a; v < b; ++v) {
+      const uint64_t r0 = c.row_ptr[v], r1 = c.row_ptr[v + 1];
+      if (r1 < r0) return set_error(MCE_EGRAPH, "row_ptr decreases at vertex %u", v);
+      if (r1 > r0 && (c.col_ind[r0] < a || c.col_ind[r1 - 1] >= b))
+        return set_error(MCE_EGRAPH, "arc of vertex %u leaves graph %u", v, gi);
+    }
+  }
+  mce_graph *h = nullptr;
+  rc = mce_graph_create(&c, opts->device, &h);
+  if (rc) return rc;
+  std::vector<uint64_t> per_root(std::max<uint32_t>(c.n, 1), 0);
+  rc = mce_graph_order(h, agg);
+  if (!rc) {
+    const double to = agg->t_order_ms, tr = agg->t_orient_ms;
+    mce_opts o = *opts;
+    o.sub_cliques = per_root.data();
+    rc = count_impl(h, &o, agg, true);
+    agg->t_order_ms = to;
+    agg->t_orient_ms = tr;
+  }
+  mce_graph_destroy(h);
+  if (rc) return rc;
+  uint64_t total = 0;
+  for (uint32_t gi = 0; gi < G; ++gi) {
+    uint64_t sum = 0;
+    for (uint32_t v = batch->vtx_off[gi]; v < batch->vtx_off[gi + 1]; ++v) sum += per_root[v];
+    counts_out[gi] = sum;
+    total += sum;
+  }
+  if (total != agg->count) return set_error(MCE_EINTERNAL, "per-graph counts do not add up");
+  return MCE_OK;
+}
+
+int mce_autotune(mce_graph *h, const mce_opts *base, double sample_fraction, mce_opts *best, char *report,
+                 uint64_t report_len) {
+  clear_error();
+  if (!h || !base || !best) return set_error(MCE_EINVAL, "null argument");
+  int rc = check_opts(base);
+  if (rc) return rc;
+  if (!h->ordered) return set_error(MCE_EINVAL, "graph is not ordered (call mce_graph_order first)");
+  if (!(sample_fraction > 0 && sample_fraction <= 1)) return set_error(MCE_EINVAL, "sample_fraction must be in (0, 1]");
+  const DevGraph &g = h->g;
+  // a stride sample of the vertices with out-degree >= 1
+  std::vector<uint64_t> split(std::max<uint32_t>(g.n, 1)), rp((size_t)g.n + 1);
+  MCE_HIP(hipSetDevice(h->device));
+  if (g.n) {
+    MCE_HIP(hipMemcpy(split.data(), g.split, sizeof(uint64_t) * g.n, hipMemcpyDeviceToHost));
+    MCE_HIP(hipMemcpy(rp.data(), g.rp, sizeof(uint64_t) * (g.n + 1), hipMemcpyDeviceToHost));
+  }
+  const uint64_t stride = std::max<uint64_t>(1, (uint64_t)(1.0 / sample_fraction + 0.5));
+  std::vector<uint32_t> sample;
+  for (uint64_t v = 0; v < g.n; v += stride)
+    if (rp[v + 1] > split[v]) sample.push_back((uint32_t)v);
+  struct Cand {
+    int lds, wpg, stack, minp, check;
+  };
+  std::vector<Cand> cands;
+  const int ldsv[] = {0, 6144, 12288, 24576, 40960};
+  const int wpgv[] = {4, 2, 1};
+  for (int l : ldsv)
+    for (int w : wpgv)
+      if ((size_t)w * (size_t)(l ? l : 0) <= 150 * 1024 && !(l == 0 && w != 4)) cands.push_back({l, w, 0, 0, 0});
+  std::string rep = "[";
+  mce_opts cur = *base;
+  cur.schedule = MCE_SCHED_GIVEN;
+  cur.order = sample.data();
+  cur.order_len = sample.size();
+  cur.sub_nodes = cur.sub_cliques = nullptr;
+  mce_result r{};
+  double best_ms = 1e300;
+  Cand bestc{base->lds_arena_bytes, base->waves_per_group ? base->waves_per_group : 4, base->stack_lds_levels,
+             base->donate_min_p, base->donate_check_nodes};
+  uint64_t ref_count = 0, ref_hash = 0;
+  bool have_ref = false;
+  auto run = [&](const Cand &cd) -> int {
+    mce_opts o = cur;
+    o.lds_arena_bytes = cd.lds;
+    o.waves_per_group = cd.wpg;
+    o.stack_lds_levels = cd.stack;
+    if (cd.minp) o.donate_min_p = cd.minp;
+    if (cd.check) o.donate_check_nodes = cd.check;
+    memset(&r, 0, sizeof(r));
+    const int e = count_impl(h, &o, &r, false);
+    if (e) return e;
+    if (!have_ref) {
+      ref_count = r.count;
+      ref_hash = r.hash;
+      have_ref = true;
+    } else if (r.count != ref_count || r.hash != ref_hash) {
+      return set_error(MCE_EINTERNAL, "results changed with the block settings");
+    }
+    char buf[256];
+    snprintf(buf, sizeof(buf), "%s{\"lds_arena_bytes\":%d,\"waves_per_group\":%d,\"stack_lds_levels\":%d,"
+             "\"donate_min_p\":%d,\"donate_check_nodes\":%d,\"ms\":%.3f}",
+             rep.size() > 1 ? "," : "", cd.lds, cd.wpg, cd.stack, cd.minp, cd.check, r.t_count_ms);
+    rep += buf;
+    if (r.t_count_ms < best_ms) {
+      best_ms = r.t_count_ms;
+      bestc = cd;
+    }
+    return MCE_OK;
+  };
+  for (const Cand &cd : cands)
+    if ((rc = run(cd))) return rc;
+  // then the stack depth kept in LDS, then (worker list) the donation thresholds
+  const Cand arena = bestc;
+  for (int st : {8, 32}) {
+    Cand cd = arena;
+    cd.stack = st;
+    if ((rc = run(cd))) return rc;
+  }
+  if (base->worklist == MCE_WL) {
+    const Cand b2 = bestc;
+    for (int mp : {4, 10})
+      for (int ck : {16, 64}) {
+        Cand cd = b2;
+        cd.minp = mp;
+        cd.check = ck;
+        if ((rc = run(cd))) return rc;
+      }
+  }
+  rep += "]";
+  *best = *base;
+  best->lds_arena_bytes = bestc.lds;
+  best->waves_per_group = bestc.wpg;
+  best->stack_lds_levels = bestc.stack;
+  if (bestc.minp) best->donate_min_p = bestc.minp;
+  if (bestc.check) best->donate_check_nodes = bestc.check;
+  if (report && report_len) {
+    snprintf(report, report_len, "%s", rep.c_str());
+  }
+  return MCE_OK;
 }
 
 }  // extern "C"

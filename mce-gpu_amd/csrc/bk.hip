@@ -90,6 +90,7 @@ struct BkParams {
   uint32_t chunk;  // later tickets carry `chunk` items
   uint32_t level;
   uint32_t lds_bytes;  // per-wave LDS arena
+  uint32_t stack_lds;  // most BK frames kept in LDS (0 = as many as fit)
   uint8_t *hbm;        // per-wave HBM arenas
   uint64_t hbm_bytes;
   unsigned long long *acc;
@@ -183,7 +184,7 @@ __device__ __forceinline__ uint64_t fixed_bytes(uint32_t p, uint32_t q, uint32_t
 // Lays out the arena for bounds (p, q, W). Returns false when the HBM arena
 // is too small.
 __device__ __forceinline__ bool place(Sub &s, uint32_t p, uint32_t q, uint32_t W, bool ipx, uint8_t *lds, uint32_t lds_bytes,
-                      uint8_t *hbm, uint64_t hbm_bytes) {
+                      uint8_t *hbm, uint64_t hbm_bytes, uint32_t stack_lds = 0) {
   const uint64_t fixed = fixed_bytes(p, q, W, ipx);
   const uint32_t FS = 16 * W + 16;
   const uint32_t D = p;
@@ -193,12 +194,14 @@ __device__ __forceinline__ bool place(Sub &s, uint32_t p, uint32_t q, uint32_t W
     base = lds;
     s.fr_lds = lds + fixed;
     s.lds_levels = (uint32_t)((lds_bytes - fixed) / FS);
+    if (stack_lds && s.lds_levels > stack_lds) s.lds_levels = stack_lds;
     s.fr_hbm = hbm;
     need = D > s.lds_levels ? (uint64_t)(D - s.lds_levels) * FS : 0;
   } else {
     base = hbm;
     s.fr_lds = lds;
     s.lds_levels = lds_bytes / FS;
+    if (stack_lds && s.lds_levels > stack_lds) s.lds_levels = stack_lds;
     s.fr_hbm = hbm + fixed;
     need = fixed + (D > s.lds_levels ? (uint64_t)(D - s.lds_levels) * FS : 0);
   }
@@ -220,15 +223,18 @@ __device__ __forceinline__ bool place(Sub &s, uint32_t p, uint32_t q, uint32_t W
 }
 
 // HBM bytes a subproblem with bounds (p, q) needs beyond the LDS arena.
-__device__ __forceinline__ uint64_t hbm_need(uint32_t p, uint32_t q, bool ipx, uint32_t lds_bytes) {
+__device__ __forceinline__ uint64_t hbm_need(uint32_t p, uint32_t q, bool ipx, uint32_t lds_bytes,
+                                             uint32_t stack_lds = 0) {
   const uint32_t W = (p + 63) / 64;
   const uint64_t fixed = fixed_bytes(p, q, W, ipx);
   const uint32_t FS = 16 * W + 16;
   if (fixed + 4ull * FS <= lds_bytes) {
-    const uint32_t lv = (uint32_t)((lds_bytes - fixed) / FS);
+    uint32_t lv = (uint32_t)((lds_bytes - fixed) / FS);
+    if (stack_lds && lv > stack_lds) lv = stack_lds;
     return p > lv ? (uint64_t)(p - lv) * FS : 0;
   }
-  const uint32_t lv = lds_bytes / FS;
+  uint32_t lv = lds_bytes / FS;
+  if (stack_lds && lv > stack_lds) lv = stack_lds;
   return fixed + (p > lv ? (uint64_t)(p - lv) * FS : 0);
 }
 
@@ -526,7 +532,7 @@ __device__ __forceinline__ bool donate_shallowest(const BkParams &pr, Sub &s, ui
       const uint32_t i = b + L;
       nx += popc64(ballot(i < xej && xe_adj<kIPX>(s, s.XL[i], c, nb, nlen)));
     }
-    if (nx > pr.slot_xcap || hbm_need(s.p, nx, kIPX, pr.lds_bytes) > pr.hbm_bytes) continue;
+    if (nx > pr.slot_xcap || hbm_need(s.p, nx, kIPX, pr.lds_bytes, pr.stack_lds) > pr.hbm_bytes) continue;
     unsigned long long pos = 0;
     if (L == 0) {
       pos = ring_claim_push(pr);
@@ -699,7 +705,7 @@ __device__ __forceinline__ uint64_t bits_fixed(uint32_t p, uint32_t q, uint32_t 
 __device__ __forceinline__ uint32_t bits_fs(uint32_t W, uint32_t WX) { return 8 * (2 * W + WX) + 16; }
 
 __device__ __forceinline__ bool place_bits(Sub &s, uint32_t p, uint32_t q, uint32_t W, uint8_t *lds, uint32_t lds_bytes, uint8_t *hbm,
-                           uint64_t hbm_bytes) {
+                           uint64_t hbm_bytes, uint32_t stack_lds = 0) {
   const uint32_t WX = (q + 63) / 64;
   const uint64_t fixed = bits_fixed(p, q, W);
   const uint32_t FS = bits_fs(W, WX);
@@ -710,12 +716,14 @@ __device__ __forceinline__ bool place_bits(Sub &s, uint32_t p, uint32_t q, uint3
     base = lds;
     s.fr_lds = lds + fixed;
     s.lds_levels = (uint32_t)((lds_bytes - fixed) / FS);
+    if (stack_lds && s.lds_levels > stack_lds) s.lds_levels = stack_lds;
     s.fr_hbm = hbm;
     need = D > s.lds_levels ? (uint64_t)(D - s.lds_levels) * FS : 0;
   } else {
     base = hbm;
     s.fr_lds = lds;
     s.lds_levels = lds_bytes / FS;
+    if (stack_lds && s.lds_levels > stack_lds) s.lds_levels = stack_lds;
     s.fr_hbm = hbm + fixed;
     need = fixed + (D > s.lds_levels ? (uint64_t)(D - s.lds_levels) * FS : 0);
   }
@@ -731,15 +739,17 @@ __device__ __forceinline__ bool place_bits(Sub &s, uint32_t p, uint32_t q, uint3
   return true;
 }
 
-__device__ __forceinline__ uint64_t bits_need(uint32_t p, uint32_t q, uint32_t lds_bytes) {
+__device__ __forceinline__ uint64_t bits_need(uint32_t p, uint32_t q, uint32_t lds_bytes, uint32_t stack_lds = 0) {
   const uint32_t W = (p + 63) / 64, WX = (q + 63) / 64;
   const uint64_t fixed = bits_fixed(p, q, W);
   const uint32_t FS = bits_fs(W, WX);
   if (fixed + 4ull * FS <= lds_bytes) {
-    const uint32_t lv = (uint32_t)((lds_bytes - fixed) / FS);
+    uint32_t lv = (uint32_t)((lds_bytes - fixed) / FS);
+    if (stack_lds && lv > stack_lds) lv = stack_lds;
     return p > lv ? (uint64_t)(p - lv) * FS : 0;
   }
-  const uint32_t lv = lds_bytes / FS;
+  uint32_t lv = lds_bytes / FS;
+  if (stack_lds && lv > stack_lds) lv = stack_lds;
   return fixed + (p > lv ? (uint64_t)(p - lv) * FS : 0);
 }
 
@@ -1159,8 +1169,8 @@ __global__ __launch_bounds__(256) void bk_kernel(BkParams pr) {
         if (p == 0) continue;
         const uint32_t W = (p + 63) / 64;
         const bool bits = kIPX && q <= kMaxXRows;
-        if (!(bits ? place_bits(s, p, q, W, lds, pr.lds_bytes, hbm, pr.hbm_bytes)
-                   : place(s, p, q, W, kIPX, lds, pr.lds_bytes, hbm, pr.hbm_bytes))) {
+        if (!(bits ? place_bits(s, p, q, W, lds, pr.lds_bytes, hbm, pr.hbm_bytes, pr.stack_lds)
+                   : place(s, p, q, W, kIPX, lds, pr.lds_bytes, hbm, pr.hbm_bytes, pr.stack_lds))) {
           defer(pr, it);
           continue;
         }
@@ -1184,7 +1194,7 @@ __global__ __launch_bounds__(256) void bk_kernel(BkParams pr) {
         const uint64_t sw = pr.split[w], ew = pr.rp[w + 1], bw = pr.rp[w];
         const uint32_t pb = (uint32_t)min(eu - su, ew - sw), qb = (uint32_t)min(eu - bu, sw - bw);
         Sub t = s;
-        if (!place(t, pb, qb, (pb + 63) / 64, kIPX, lds, pr.lds_bytes, hbm, pr.hbm_bytes)) {
+        if (!place(t, pb, qb, (pb + 63) / 64, kIPX, lds, pr.lds_bytes, hbm, pr.hbm_bytes, pr.stack_lds)) {
           defer(pr, it);
           continue;
         }
@@ -1199,8 +1209,8 @@ __global__ __launch_bounds__(256) void bk_kernel(BkParams pr) {
         // re-lay the arena for the actual sizes
         const uint32_t W = (p + 63) / 64;
         const bool bits = kIPX && q <= kMaxXRows;
-        if (!(bits ? place_bits(s, p, q, W, lds, pr.lds_bytes, hbm, pr.hbm_bytes)
-                   : place(s, p, q, W, kIPX, lds, pr.lds_bytes, hbm, pr.hbm_bytes))) {
+        if (!(bits ? place_bits(s, p, q, W, lds, pr.lds_bytes, hbm, pr.hbm_bytes, pr.stack_lds)
+                   : place(s, p, q, W, kIPX, lds, pr.lds_bytes, hbm, pr.hbm_bytes, pr.stack_lds))) {
           defer(pr, it);
           continue;
         }
@@ -1242,7 +1252,7 @@ __global__ __launch_bounds__(256) void bk_kernel(BkParams pr) {
         const uint64_t rmix = uniform64(*(const uint64_t *)(slot + 24));
         if (kIPX && kind == kTaskBits) {
           const bool okb = p <= pr.slot_pmax && W == (p + 63) / 64 && nx <= pr.slot_xcap && nx <= kMaxXRows &&
-                           place_bits(s, p, nx, W, lds, pr.lds_bytes, hbm, pr.hbm_bytes);
+                           place_bits(s, p, nx, W, lds, pr.lds_bytes, hbm, pr.hbm_bytes, pr.stack_lds);
           uint64_t P0 = 0, X0 = 0, E0 = 0;
           if (okb) {
             const uint64_t *sp = (const uint64_t *)(slot + pr.slot_oP);
@@ -1288,7 +1298,7 @@ __global__ __launch_bounds__(256) void bk_kernel(BkParams pr) {
           continue;
         }
         const bool ok = kind == kTaskList && p <= pr.slot_pmax && W == (p + 63) / 64 && nx <= pr.slot_xcap &&
-                        place(s, p, nx, W, kIPX, lds, pr.lds_bytes, hbm, pr.hbm_bytes);
+                        place(s, p, nx, W, kIPX, lds, pr.lds_bytes, hbm, pr.hbm_bytes, pr.stack_lds);
         uint64_t P0 = 0, X0 = 0;
         if (ok) {
           const uint64_t *sp = (const uint64_t *)(slot + pr.slot_oP);
@@ -1373,6 +1383,11 @@ __global__ __launch_bounds__(256) void bk_kernel(BkParams pr) {
     if (hist_lds[i]) atomicAdd(&pr.hist[min(i, pr.hist_len - 1)], hist_lds[i]);
 }
 
+__global__ void reset_kernel(unsigned long long *acc, uint32_t n, uint32_t *ticket) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) acc[i] = 0;
+  if (threadIdx.x == 0) *ticket = 0;
+}
+
 __global__ void ring_init(unsigned long long *seq, uint32_t cap, unsigned long long *ctl,
                           unsigned long long outstanding) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) seq[i] = i;
@@ -1381,7 +1396,8 @@ __global__ void ring_init(unsigned long long *seq, uint32_t cap, unsigned long l
 
 // Largest HBM arena any item needs, and how many need more than `cap`.
 __global__ void need_kernel(const uint64_t *rp, const uint64_t *split, const void *items, uint64_t n_items,
-                            uint32_t level, bool ipx, uint32_t lds_bytes, uint64_t cap, unsigned long long *out) {
+                            uint32_t level, bool ipx, uint32_t lds_bytes, uint32_t stack_lds, uint64_t cap,
+                            unsigned long long *out) {
   unsigned long long mx = 0, over = 0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_items; i += stride) {
@@ -1399,10 +1415,11 @@ __global__ void need_kernel(const uint64_t *rp, const uint64_t *split, const voi
     // bounds, then re-lay for the actual sizes (up to four more HBM frames)
     uint64_t need;
     if (level == 2)
-      need = max(hbm_need(p, q, ipx, lds_bytes), ipx && q <= kMaxXRows ? bits_need(p, q, lds_bytes) : 0ull) +
+      need = max(hbm_need(p, q, ipx, lds_bytes, stack_lds),
+                 ipx && q <= kMaxXRows ? bits_need(p, q, lds_bytes, stack_lds) : 0ull) +
              4ull * (8 * (2 * ((p + 63) / 64) + (q + 63) / 64) + 16);
     else
-      need = ipx && q <= kMaxXRows ? bits_need(p, q, lds_bytes) : hbm_need(p, q, ipx, lds_bytes);
+      need = ipx && q <= kMaxXRows ? bits_need(p, q, lds_bytes, stack_lds) : hbm_need(p, q, ipx, lds_bytes, stack_lds);
     mx = max(mx, (unsigned long long)need);
     over += need > cap;
   }
@@ -1521,8 +1538,8 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   unsigned long long *need = acc + kAccN;
   MCE_HIP(hipMemsetAsync(acc, 0, sizeof(unsigned long long) * (kAccN + 2), s));
   const uint64_t cap = 16ull << 20;
-  need_kernel<<<1024, 256, 0, s>>>(g.rp, g.split, d_items, n_items, (uint32_t)cfg.level, cfg.ipx, lds_wave, cap,
-                                   need);
+  need_kernel<<<1024, 256, 0, s>>>(g.rp, g.split, d_items, n_items, (uint32_t)cfg.level, cfg.ipx, lds_wave,
+                                   cfg.stack_lds_levels, cap, need);
   unsigned long long hneed[2];
   MCE_HIP(hipMemcpyAsync(hneed, need, sizeof(hneed), hipMemcpyDeviceToHost, s));
   MCE_HIP(hipStreamSynchronize(s));
@@ -1564,6 +1581,7 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   pr.chunk = 8;
   pr.level = (uint32_t)cfg.level;
   pr.lds_bytes = lds_wave;
+  pr.stack_lds = cfg.stack_lds_levels;
   pr.hbm = hbm;
   pr.hbm_bytes = arena;
   pr.acc = acc;
@@ -1603,15 +1621,40 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
     if ((rc = dmalloc((void **)&pr.ctl, sizeof(unsigned long long) * kCtlWords))) return rc;
     if ((rc = dmalloc((void **)&pr.seq, sizeof(unsigned long long) * pr.ring_cap))) return rc;
     if ((rc = dmalloc((void **)&pr.ring, pr.slot_bytes * pr.ring_cap))) return rc;
-    ring_init<<<64, 256, 0, s>>>(pr.seq, pr.ring_cap, pr.ctl, n_tickets);
+    if (!cfg.capture) ring_init<<<64, 256, 0, s>>>(pr.seq, pr.ring_cap, pr.ctl, n_tickets);
   }
 
   hipEvent_t e0, e1;
   MCE_HIP(hipEventCreate(&e0));
   MCE_HIP(hipEventCreate(&e1));
   MCE_HIP(hipMemsetAsync(ticket, 0, sizeof(uint32_t), s));
-  MCE_HIP(hipEventRecord(e0, s));
-  if ((rc = dispatch(cfg.ipx, cfg.wl, stat, false, nullptr, pr, groups, wpg, lds, s))) return rc;
+  if (cfg.capture) {
+    // one persistent launch, captured into a graph: counters reset by a kernel
+    // (no memset nodes), then the ring set-up and the enumeration kernel
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    MCE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    reset_kernel<<<1, 64, 0, s>>>(acc, kAccN, ticket);
+    if (cfg.wl) ring_init<<<64, 256, 0, s>>>(pr.seq, pr.ring_cap, pr.ctl, n_tickets);
+    rc = dispatch(cfg.ipx, cfg.wl, stat, false, nullptr, pr, groups, wpg, lds, s);
+    const hipError_t ec = hipStreamEndCapture(s, &graph);
+    if (rc) return rc;
+    MCE_HIP(ec);
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+      hipGraphDestroy(graph);
+      return hip_fail(ei, "hipGraphInstantiate");
+    }
+    MCE_HIP(hipEventRecord(e0, s));
+    const hipError_t el = hipGraphLaunch(exec, s);
+    MCE_HIP(hipStreamSynchronize(s));
+    hipGraphExecDestroy(exec);
+    hipGraphDestroy(graph);
+    MCE_HIP(el);
+  } else {
+    MCE_HIP(hipEventRecord(e0, s));
+    if ((rc = dispatch(cfg.ipx, cfg.wl, stat, false, nullptr, pr, groups, wpg, lds, s))) return rc;
+  }
   // items that did not fit: rerun them with arenas sized for the largest
   unsigned long long novf = 0;
   if (hneed[1]) {

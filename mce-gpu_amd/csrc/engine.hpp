@@ -50,9 +50,11 @@ struct BkLaunch {
   uint32_t waves_per_group = 4;
   uint32_t groups = 0; // 0 = fill the device
   uint32_t lds_bytes = 0;
+  uint32_t stack_lds_levels = 0;
   uint64_t hbm_bytes = 0;
   uint32_t donate_min_p = 10;
   uint32_t donate_check_nodes = 64;
+  bool capture = false;  // run the enumeration launch as a captured hipGraph
 };
 
 struct BkOut {
