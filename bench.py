@@ -184,7 +184,8 @@ def main():
     if not args.no_cpu:
         oracle = _import("mce_oracle", os.path.join(ROOT, "oracle", "oracle.py"))
         sg, order, what = sample_workload(mce, g, off, args.cpu_sample)
-        threads = os.cpu_count() or 1
+        # the host share this process may use (OMP_NUM_THREADS where the host sets it)
+        threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or os.cpu_count() or 1
         t = time.perf_counter()
         ref = oracle.mce(sg, level=args.level, ipx=args.induced == 2, threads=threads, order=order)
         cpu_s = time.perf_counter() - t
