@@ -89,13 +89,18 @@ typedef struct mce_opts {
   int device;   /* HIP device ordinal; the engine uses exactly one GPU */
 
   /* block settings (0 = automatic) */
-  int waves_per_group;     /* wave64s per workgroup (1..16)                 */
-  int groups_per_cu;       /* resident workgroups per CU                    */
-  int lds_arena_bytes;     /* LDS bytes per wave: bitmap cutoff for LDS     */
+  int waves_per_group;     /* wave64s per workgroup (1..12); the workgroup's
+                              waves share one LDS pool                     */
+  int groups_per_cu;       /* resident workgroups per CU (1..16)            */
+  int lds_arena_bytes;     /* LDS pool bytes per wave; a subproblem takes
+                              what its bitmaps need (0 = the CU's LDS)     */
   int hbm_arena_bytes;     /* HBM spill arena per wave for larger problems  */
   int stack_lds_levels;    /* BK frames kept in LDS before spilling to HBM  */
   int donate_min_p;        /* donate a child only if |P| >= this (SZTH)     */
   int donate_check_nodes;  /* tree nodes between polls of the idle count    */
+  int donate_age_us;       /* a task donates only after running this long
+                              (0 = 1000 us): short tasks finish sooner
+                              than a copy of their bit matrix pays back    */
 
   int schedule;            /* enum mce_schedule                              */
   const uint32_t *order;   /* MCE_SCHED_GIVEN: first-level vertex order      */
@@ -259,10 +264,12 @@ int mce_run(const mce_csr *csr, const mce_opts *opts, mce_result *res);
 int mce_run_batch(const mce_batch *batch, const mce_opts *opts,
                   uint64_t *counts_out, mce_result *agg);
 
-/* Autotuner: times candidate block settings (LDS arena per wave, waves per
- * workgroup, BK frames kept in LDS, donation thresholds) on a stride sample
- * of the first-level subproblems of an ordered graph and writes the fastest
- * into *best; report (nullable) receives a JSON list of every trial. Results
+/* Autotuner: times candidate block settings (waves per workgroup and
+ * workgroups per CU, which set how many waves share one LDS pool; the BK
+ * frame levels a subproblem asks the pool for; donation thresholds) on a
+ * stride sample of the first-level subproblems of an ordered graph (at least
+ * 64K of them, or all), heaviest first, and writes the fastest into *best;
+ * report (nullable) receives a JSON list of every trial. Results
  * do not depend on the settings; a trial that changes them is an error.
  * No reference counterpart (its block size is a fixed rule, mce.cuh:125-126). */
 int mce_autotune(mce_graph *g, const mce_opts *base, double sample_fraction,

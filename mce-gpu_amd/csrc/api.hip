@@ -78,9 +78,10 @@ int check_opts(const mce_opts *o) {
     return set_error(MCE_EINVAL, "No breakdown for kernels without worker list");
   if (o->task == MCE_TASK_DONOR_EVAL && o->worklist == MCE_NOWL)
     return set_error(MCE_EINVAL, "No donation for kernels without worker list");
-  if (o->waves_per_group < 0 || o->waves_per_group > 4) return set_error(MCE_EINVAL, "waves_per_group must be 0..4");
+  if (o->waves_per_group < 0 || o->waves_per_group > 12) return set_error(MCE_EINVAL, "waves_per_group must be 0..12");
+  if (o->groups_per_cu > 16) return set_error(MCE_EINVAL, "groups_per_cu must be 0..16");
   if (o->groups_per_cu < 0 || o->lds_arena_bytes < 0 || o->hbm_arena_bytes < 0 || o->stack_lds_levels < 0 ||
-      o->donate_min_p < 0 || o->donate_check_nodes < 0)
+      o->donate_min_p < 0 || o->donate_check_nodes < 0 || o->donate_age_us < 0)
     return set_error(MCE_EINVAL, "negative block setting");
   if (o->lds_arena_bytes && (o->lds_arena_bytes < 1024 || o->lds_arena_bytes % 8))
     return set_error(MCE_EINVAL, "lds_arena_bytes must be a multiple of 8 and >= 1024");
@@ -355,23 +356,19 @@ static int count_impl(mce_graph *h, const mce_opts *opts, mce_result *res, bool 
   cfg.ipx = opts->induced == MCE_IPX;
   cfg.wl = opts->worklist == MCE_WL;
   cfg.task = opts->task;
-  cfg.waves_per_group = opts->waves_per_group ? (uint32_t)opts->waves_per_group : 4;
+  cfg.waves_per_group = (uint32_t)opts->waves_per_group;
+  cfg.groups_per_cu = (uint32_t)opts->groups_per_cu;
   cfg.lds_bytes = (uint32_t)opts->lds_arena_bytes;
   cfg.stack_lds_levels = (uint32_t)opts->stack_lds_levels;
   cfg.donate_min_p = (uint32_t)opts->donate_min_p;
   cfg.donate_check_nodes = opts->donate_check_nodes ? (uint32_t)opts->donate_check_nodes : 64;
+  cfg.donate_age_us = opts->donate_age_us ? (uint32_t)opts->donate_age_us : 1000;
   cfg.capture = capture;
   if (heavy_tail && !opts->donate_check_nodes) {
     // thresholds set from the cost model: a heavy tail wants early, fine donation
     cfg.donate_check_nodes = 16;
     cfg.donate_min_p = std::min<uint32_t>(cfg.donate_min_p, 6);
-  }
-  if (opts->groups_per_cu) {
-    int dev = 0;
-    hipDeviceProp_t prop;
-    hipGetDevice(&dev);
-    hipGetDeviceProperties(&prop, dev);
-    cfg.groups = (uint32_t)opts->groups_per_cu * (uint32_t)prop.multiProcessorCount;
+    if (!opts->donate_age_us) cfg.donate_age_us = 100;
   }
   BkOut out;
   const bool want_waves = opts->task == MCE_TASK_LB_EVAL || opts->task == MCE_TASK_DONOR_EVAL ||
@@ -507,38 +504,44 @@ int mce_autotune(mce_graph *h, const mce_opts *base, double sample_fraction, mce
     MCE_HIP(hipMemcpy(split.data(), g.split, sizeof(uint64_t) * g.n, hipMemcpyDeviceToHost));
     MCE_HIP(hipMemcpy(rp.data(), g.rp, sizeof(uint64_t) * (g.n + 1), hipMemcpyDeviceToHost));
   }
-  const uint64_t stride = std::max<uint64_t>(1, (uint64_t)(1.0 / sample_fraction + 0.5));
+  // a stride sample of the vertices with out-degree >= 1, at least 64K of
+  // them (or all) so each wave sees many subproblems, heaviest first as the
+  // plain schedule runs them
+  uint64_t stride = std::max<uint64_t>(1, (uint64_t)(1.0 / sample_fraction + 0.5));
+  if (g.n / stride < 65536) stride = std::max<uint64_t>(1, g.n / 65536);
   std::vector<uint32_t> sample;
   for (uint64_t v = 0; v < g.n; v += stride)
     if (rp[v + 1] > split[v]) sample.push_back((uint32_t)v);
+  std::stable_sort(sample.begin(), sample.end(), [&](uint32_t x, uint32_t y) {
+    return rp[x + 1] - split[x] > rp[y + 1] - split[y];
+  });
   struct Cand {
-    int lds, wpg, stack, minp, check;
+    int wpg, gpc, stack, minp, check, age;
   };
-  std::vector<Cand> cands;
-  const int ldsv[] = {0, 6144, 12288, 24576, 40960};
-  const int wpgv[] = {4, 2, 1};
-  for (int l : ldsv)
-    for (int w : wpgv)
-      if ((size_t)w * (size_t)(l ? l : 0) <= 150 * 1024 && !(l == 0 && w != 4)) cands.push_back({l, w, 0, 0, 0});
+  // waves per workgroup x workgroups per CU; each workgroup shares its LDS
+  const Cand grid[] = {{0, 0, 0, 0, 0, 0}, {4, 1, 0, 0, 0, 0}, {8, 1, 0, 0, 0, 0}, {12, 1, 0, 0, 0, 0},
+                       {8, 2, 0, 0, 0, 0}, {6, 2, 0, 0, 0, 0}};
   std::string rep = "[";
   mce_opts cur = *base;
   cur.schedule = MCE_SCHED_GIVEN;
   cur.order = sample.data();
   cur.order_len = sample.size();
   cur.sub_nodes = cur.sub_cliques = nullptr;
+  cur.lds_arena_bytes = 0;
   mce_result r{};
   double best_ms = 1e300;
-  Cand bestc{base->lds_arena_bytes, base->waves_per_group ? base->waves_per_group : 4, base->stack_lds_levels,
-             base->donate_min_p, base->donate_check_nodes};
+  Cand bestc{base->waves_per_group, base->groups_per_cu, base->stack_lds_levels, base->donate_min_p,
+             base->donate_check_nodes, base->donate_age_us};
   uint64_t ref_count = 0, ref_hash = 0;
   bool have_ref = false;
   auto run = [&](const Cand &cd) -> int {
     mce_opts o = cur;
-    o.lds_arena_bytes = cd.lds;
     o.waves_per_group = cd.wpg;
+    o.groups_per_cu = cd.gpc;
     o.stack_lds_levels = cd.stack;
     if (cd.minp) o.donate_min_p = cd.minp;
     if (cd.check) o.donate_check_nodes = cd.check;
+    if (cd.age) o.donate_age_us = cd.age;
     memset(&r, 0, sizeof(r));
     const int e = count_impl(h, &o, &r, false);
     if (e) return e;
@@ -550,9 +553,10 @@ int mce_autotune(mce_graph *h, const mce_opts *base, double sample_fraction, mce
       return set_error(MCE_EINTERNAL, "results changed with the block settings");
     }
     char buf[256];
-    snprintf(buf, sizeof(buf), "%s{\"lds_arena_bytes\":%d,\"waves_per_group\":%d,\"stack_lds_levels\":%d,"
-             "\"donate_min_p\":%d,\"donate_check_nodes\":%d,\"ms\":%.3f}",
-             rep.size() > 1 ? "," : "", cd.lds, cd.wpg, cd.stack, cd.minp, cd.check, r.t_count_ms);
+    snprintf(buf, sizeof(buf), "%s{\"waves_per_group\":%d,\"groups_per_cu\":%d,\"stack_lds_levels\":%d,"
+             "\"donate_min_p\":%d,\"donate_check_nodes\":%d,\"donate_age_us\":%d,\"waves\":%u,\"ms\":%.3f}",
+             rep.size() > 1 ? "," : "", cd.wpg, cd.gpc, cd.stack, cd.minp, cd.check, cd.age, r.num_waves,
+             r.t_count_ms);
     rep += buf;
     if (r.t_count_ms < best_ms) {
       best_ms = r.t_count_ms;
@@ -560,32 +564,35 @@ int mce_autotune(mce_graph *h, const mce_opts *base, double sample_fraction, mce
     }
     return MCE_OK;
   };
-  for (const Cand &cd : cands)
+  for (const Cand &cd : grid)
     if ((rc = run(cd))) return rc;
-  // then the stack depth kept in LDS, then (worker list) the donation thresholds
-  const Cand arena = bestc;
-  for (int st : {8, 32}) {
-    Cand cd = arena;
+  // then the frame levels a subproblem asks the pool for, then (worker list)
+  // the donation thresholds
+  const Cand shape = bestc;
+  for (int st : {8, 48}) {
+    Cand cd = shape;
     cd.stack = st;
     if ((rc = run(cd))) return rc;
   }
   if (base->worklist == MCE_WL) {
     const Cand b2 = bestc;
-    for (int mp : {4, 10})
-      for (int ck : {16, 64}) {
+    for (int mp : {4, 10, 24})
+      for (int age : {100, 1000, 10000}) {
         Cand cd = b2;
         cd.minp = mp;
-        cd.check = ck;
+        cd.age = age;
         if ((rc = run(cd))) return rc;
       }
   }
   rep += "]";
   *best = *base;
-  best->lds_arena_bytes = bestc.lds;
+  best->lds_arena_bytes = 0;
   best->waves_per_group = bestc.wpg;
+  best->groups_per_cu = bestc.gpc;
   best->stack_lds_levels = bestc.stack;
   if (bestc.minp) best->donate_min_p = bestc.minp;
   if (bestc.check) best->donate_check_nodes = bestc.check;
+  if (bestc.age) best->donate_age_us = bestc.age;
   if (report && report_len) {
     snprintf(report, report_len, "%s", rep.c_str());
   }

@@ -73,6 +73,8 @@ constexpr uint32_t kSlotHeader = 64;  // bytes
 // IPX subproblems with at most this many outside-X rows keep them as a bitmap
 // over row indices (one word per lane, like P and X); larger ones use lists.
 constexpr uint32_t kMaxXRows = 4096;
+// Most waves in one workgroup (the launch bound of the enumeration kernel).
+constexpr uint32_t kMaxWpg = 12;
 enum TaskKind : uint32_t { kTaskList = 0, kTaskBits = 1 };
 // 100 MHz realtime ticks an idle wave waits without receiving work before it
 // reports a termination failure and exits: 120 s.
@@ -89,8 +91,9 @@ struct BkParams {
   uint64_t t1;     // tickets below t1 carry one item each
   uint32_t chunk;  // later tickets carry `chunk` items
   uint32_t level;
-  uint32_t lds_bytes;  // per-wave LDS arena
-  uint32_t stack_lds;  // most BK frames kept in LDS (0 = as many as fit)
+  uint32_t pool_bytes;  // LDS pool shared by the waves of a workgroup
+  uint32_t stack_lds;   // most BK frames kept in LDS (0 = as many as fit)
+  uint32_t want_levels; // frame levels a subproblem asks the pool for
   uint8_t *hbm;        // per-wave HBM arenas
   uint64_t hbm_bytes;
   unsigned long long *acc;
@@ -117,6 +120,7 @@ struct BkParams {
   uint32_t slot_oP, slot_oS, slot_oA, slot_oXI, slot_oXR;  // byte offsets in a slot
   uint32_t donate_min_p;
   uint32_t donate_check;
+  uint64_t donate_age;  // 100 MHz realtime ticks a task runs before it may donate
 };
 
 struct Sub {
@@ -132,6 +136,7 @@ struct Sub {
   uint32_t FS;
   uint32_t p, W, q, xe0;
   uint32_t WX;  // bitmap path: words of the outside-X row bitmap
+  uint32_t where;  // bitmap path: bit 0 S and A in LDS, bit 1 outside rows in LDS
   uint32_t rsize;
   uint32_t root;  // first-level vertex the subproblem belongs to
   uint64_t rmix;
@@ -146,6 +151,7 @@ struct WaveAcc {
   uint64_t sub_cliques = 0;
   int32_t budget = 0;
   uint64_t last_poll = 0;
+  uint64_t t_task = 0;  // realtime at the start of the current task
 };
 
 template <int kStat>
@@ -172,6 +178,37 @@ struct Clock {
 
 __device__ __forceinline__ uint64_t align8(uint64_t x) { return (x + 7) & ~7ull; }
 
+// Typed arena accesses: kL picks LDS (ds_*) over global memory (global_*).
+// Arena pointers are generic because a subproblem's placement is chosen at
+// run time; the bitmap solver is instantiated per placement so that its hot
+// loop issues the instruction of the memory it touches instead of flat_*
+// (which waits on both the LDS and the vector-memory counters).
+typedef __attribute__((address_space(3))) uint64_t lds_u64;
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(1))) uint64_t glb_u64;
+typedef __attribute__((address_space(1))) uint32_t glb_u32;
+
+template <bool kL>
+__device__ __forceinline__ uint64_t ld64(const uint64_t *p, uint64_t i) {
+  if constexpr (kL) return ((const lds_u64 *)p)[i];
+  else return ((const glb_u64 *)p)[i];
+}
+template <bool kL>
+__device__ __forceinline__ uint32_t ld32(const uint32_t *p, uint64_t i) {
+  if constexpr (kL) return ((const lds_u32 *)p)[i];
+  else return ((const glb_u32 *)p)[i];
+}
+template <bool kL>
+__device__ __forceinline__ void st64(uint64_t *p, uint64_t i, uint64_t v) {
+  if constexpr (kL) ((lds_u64 *)p)[i] = v;
+  else ((glb_u64 *)p)[i] = v;
+}
+template <bool kL>
+__device__ __forceinline__ void st32(uint32_t *p, uint64_t i, uint32_t v) {
+  if constexpr (kL) ((lds_u32 *)p)[i] = v;
+  else ((glb_u32 *)p)[i] = v;
+}
+
 __device__ __forceinline__ uint8_t *frame_at(const Sub &s, uint32_t d) {
   return d < s.lds_levels ? s.fr_lds + (uint64_t)d * s.FS : s.fr_hbm + (uint64_t)(d - s.lds_levels) * s.FS;
 }
@@ -190,7 +227,7 @@ __device__ __forceinline__ bool place(Sub &s, uint32_t p, uint32_t q, uint32_t W
   const uint32_t D = p;
   uint8_t *base;
   uint64_t need;
-  if (fixed + 4ull * FS <= lds_bytes) {
+  if (fixed + FS <= lds_bytes) {
     base = lds;
     s.fr_lds = lds + fixed;
     s.lds_levels = (uint32_t)((lds_bytes - fixed) / FS);
@@ -223,19 +260,117 @@ __device__ __forceinline__ bool place(Sub &s, uint32_t p, uint32_t q, uint32_t W
 }
 
 // HBM bytes a subproblem with bounds (p, q) needs beyond the LDS arena.
-__device__ __forceinline__ uint64_t hbm_need(uint32_t p, uint32_t q, bool ipx, uint32_t lds_bytes,
-                                             uint32_t stack_lds = 0) {
+__device__ __forceinline__ uint64_t hbm_need(uint32_t p, uint32_t q, bool ipx) {
   const uint32_t W = (p + 63) / 64;
-  const uint64_t fixed = fixed_bytes(p, q, W, ipx);
-  const uint32_t FS = 16 * W + 16;
-  if (fixed + 4ull * FS <= lds_bytes) {
-    uint32_t lv = (uint32_t)((lds_bytes - fixed) / FS);
-    if (stack_lds && lv > stack_lds) lv = stack_lds;
-    return p > lv ? (uint64_t)(p - lv) * FS : 0;
+  return fixed_bytes(p, q, W, ipx) + (uint64_t)p * (16 * W + 16);
+}
+
+// LDS bytes a list-path subproblem asks the pool for.
+__device__ __forceinline__ uint32_t list_want(uint32_t p, uint32_t q, uint32_t W, bool ipx, uint32_t levels) {
+  return (uint32_t)min<uint64_t>(fixed_bytes(p, q, W, ipx) + (uint64_t)(16 * W + 16) * min(p, levels), 0xFFFFFFFFull);
+}
+
+// Workgroup LDS pool: granules of kGranule bytes handed to waves for the
+// lifetime of one subproblem, first fit, under a workgroup spin lock held by
+// lane 0 of the asking wave. A wave never waits for LDS while holding any, so
+// the lock is the only wait and every holder runs to release.
+constexpr uint32_t kGranule = 512;
+constexpr uint32_t kPoolWords = 5;  // up to 320 granules (160 KiB)
+constexpr uint32_t kPoolHeader = 64;
+
+struct Pool {
+  uint64_t *mask;  // bit set: granule taken (bits past ngran stay set)
+  uint32_t *lock;
+  uint8_t *base;
+  uint32_t ngran;
+};
+
+struct Region {
+  uint8_t *ptr;
+  uint32_t bytes;
+  uint32_t g0, gk;
+};
+
+__device__ __forceinline__ void pool_lock(uint32_t *lock) {
+  while (true) {
+    uint32_t e = 0;
+    if (__hip_atomic_compare_exchange_strong(lock, &e, 1u, __ATOMIC_ACQUIRE, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP))
+      return;
+    __builtin_amdgcn_s_sleep(1);
   }
-  uint32_t lv = lds_bytes / FS;
-  if (stack_lds && lv > stack_lds) lv = stack_lds;
-  return fixed + (p > lv ? (uint64_t)(p - lv) * FS : 0);
+}
+
+__device__ __forceinline__ void pool_unlock(uint32_t *lock) {
+  __hip_atomic_store(lock, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// First run of k clear bits, or kNone.
+__device__ __forceinline__ uint32_t pool_find(const Pool &pl, uint32_t k) {
+  if (k == 0 || k > pl.ngran) return kNone;
+  uint32_t i = 0;
+  while (i + k <= pl.ngran) {
+    uint32_t w = i >> 6;
+    uint64_t x = ~pl.mask[w] & (~0ull << (i & 63));
+    while (!x) {
+      if (++w >= kPoolWords) return kNone;
+      x = ~pl.mask[w];
+    }
+    i = w * 64 + ctz64(x);
+    if (i + k > pl.ngran) return kNone;
+    uint64_t y = pl.mask[w] & (~0ull << (i & 63));
+    while (!y && ++w < kPoolWords) y = pl.mask[w];
+    const uint32_t j = y ? w * 64 + ctz64(y) : kPoolWords * 64;
+    if (j - i >= k) return i;
+    i = j;
+  }
+  return kNone;
+}
+
+__device__ __forceinline__ void pool_mark(const Pool &pl, uint32_t i, uint32_t k, bool set) {
+  while (k) {
+    const uint32_t w = i >> 6, b = i & 63, n = min(k, 64u - b);
+    const uint64_t m = (n == 64 ? ~0ull : ((1ull << n) - 1)) << b;
+    pl.mask[w] = set ? (pl.mask[w] | m) : (pl.mask[w] & ~m);
+    i += n;
+    k -= n;
+  }
+}
+
+// The largest of want, mid, low (bytes, tried in that order) that fits now;
+// an empty region when none does.
+__device__ __forceinline__ Region pool_take(const Pool &pl, uint32_t want, uint32_t mid, uint32_t low) {
+  uint32_t g0 = kNone, gk = 0;
+  if (lane_id() == 0) {
+    pool_lock(pl.lock);
+    const uint32_t k1 = (uint32_t)min<uint64_t>(((uint64_t)want + kGranule - 1) / kGranule, 0xFFFFu);
+    const uint32_t k2 = (uint32_t)min<uint64_t>(((uint64_t)mid + kGranule - 1) / kGranule, 0xFFFFu);
+    const uint32_t k3 = (uint32_t)min<uint64_t>(((uint64_t)low + kGranule - 1) / kGranule, 0xFFFFu);
+    uint32_t g = kNone, k = 0;
+    if (k1) g = pool_find(pl, k = k1);
+    if (g == kNone && k2) g = pool_find(pl, k = k2);
+    if (g == kNone && k3) g = pool_find(pl, k = k3);
+    if (g != kNone) {
+      pool_mark(pl, g, k, true);
+      g0 = g;
+      gk = k;
+    }
+    pool_unlock(pl.lock);
+  }
+  Region r;
+  r.g0 = uniform(g0);
+  r.gk = uniform(gk);
+  r.ptr = pl.base + (r.gk ? (uint64_t)r.g0 * kGranule : 0);
+  r.bytes = r.gk * kGranule;
+  return r;
+}
+
+__device__ __forceinline__ void pool_put(const Pool &pl, const Region &r) {
+  if (r.gk && lane_id() == 0) {
+    pool_lock(pl.lock);
+    pool_mark(pl, r.g0, r.gk, false);
+    pool_unlock(pl.lock);
+  }
 }
 
 __device__ __forceinline__ const uint64_t *row_of(const Sub &s, uint32_t piv) {
@@ -532,7 +667,7 @@ __device__ __forceinline__ bool donate_shallowest(const BkParams &pr, Sub &s, ui
       const uint32_t i = b + L;
       nx += popc64(ballot(i < xej && xe_adj<kIPX>(s, s.XL[i], c, nb, nlen)));
     }
-    if (nx > pr.slot_xcap || hbm_need(s.p, nx, kIPX, pr.lds_bytes, pr.stack_lds) > pr.hbm_bytes) continue;
+    if (nx > pr.slot_xcap || hbm_need(s.p, nx, kIPX) > pr.hbm_bytes) continue;
     unsigned long long pos = 0;
     if (L == 0) {
       pos = ring_claim_push(pr);
@@ -623,7 +758,8 @@ __device__ __forceinline__ void bk_solve(const BkParams &pr, Sub &s, uint64_t cP
       if (wa.nodes - wa.last_poll >= pr.donate_check) {
         wa.last_poll = wa.nodes;
         clk.to(kBdQueue);
-        wa.budget = donation_budget(pr);
+        // only a task that has run for a while gives work away
+        wa.budget = __builtin_amdgcn_s_memrealtime() - wa.t_task >= pr.donate_age ? donation_budget(pr) : 0;
       }
       if (wa.budget > 0) {
         clk.to(kBdQueue);
@@ -704,53 +840,75 @@ __device__ __forceinline__ uint64_t bits_fixed(uint32_t p, uint32_t q, uint32_t 
 
 __device__ __forceinline__ uint32_t bits_fs(uint32_t W, uint32_t WX) { return 8 * (2 * W + WX) + 16; }
 
-__device__ __forceinline__ bool place_bits(Sub &s, uint32_t p, uint32_t q, uint32_t W, uint8_t *lds, uint32_t lds_bytes, uint8_t *hbm,
-                           uint64_t hbm_bytes, uint32_t stack_lds = 0) {
+// Lays out a bitmap subproblem over an LDS region of lds_bytes (taken from
+// the workgroup pool) and the wave's HBM arena. In order of preference: all
+// of S, A, the outside rows and the first frames in LDS; S, A and the first
+// frames in LDS with the outside rows in HBM; everything but the frames the
+// region holds in HBM. Frames past the LDS levels continue in HBM.
+__device__ __forceinline__ bool place_bits(Sub &s, uint32_t p, uint32_t q, uint32_t W, uint8_t *lds, uint32_t lds_bytes,
+                                           uint8_t *hbm, uint64_t hbm_bytes, uint32_t stack_lds = 0) {
   const uint32_t WX = (q + 63) / 64;
-  const uint64_t fixed = bits_fixed(p, q, W);
+  const uint64_t sa = align8(4ull * p) + 8ull * p * W, xr = 8ull * q * W;
   const uint32_t FS = bits_fs(W, WX);
-  const uint32_t D = p;
+  uint64_t need, in_lds;
+  uint8_t *xbase;
   uint8_t *base;
-  uint64_t need;
-  if (fixed + 4ull * FS <= lds_bytes) {
+  if (sa + xr + FS <= lds_bytes) {
     base = lds;
-    s.fr_lds = lds + fixed;
-    s.lds_levels = (uint32_t)((lds_bytes - fixed) / FS);
-    if (stack_lds && s.lds_levels > stack_lds) s.lds_levels = stack_lds;
+    xbase = lds + sa;
+    in_lds = sa + xr;
     s.fr_hbm = hbm;
-    need = D > s.lds_levels ? (uint64_t)(D - s.lds_levels) * FS : 0;
+    need = 0;
+    s.where = 3;
+  } else if (sa + FS <= lds_bytes) {
+    base = lds;
+    xbase = hbm;
+    in_lds = sa;
+    s.fr_hbm = hbm + xr;
+    need = xr;
+    s.where = 1;
   } else {
     base = hbm;
-    s.fr_lds = lds;
-    s.lds_levels = lds_bytes / FS;
-    if (stack_lds && s.lds_levels > stack_lds) s.lds_levels = stack_lds;
-    s.fr_hbm = hbm + fixed;
-    need = fixed + (D > s.lds_levels ? (uint64_t)(D - s.lds_levels) * FS : 0);
+    xbase = hbm + sa;
+    in_lds = 0;
+    s.fr_hbm = hbm + sa + xr;
+    need = sa + xr;
+    s.where = 0;
   }
+  s.fr_lds = lds + in_lds;
+  s.lds_levels = (uint32_t)((lds_bytes - in_lds) / FS);
+  if (stack_lds && s.lds_levels > stack_lds) s.lds_levels = stack_lds;
+  need += p > s.lds_levels ? (uint64_t)(p - s.lds_levels) * FS : 0;
   if (need > hbm_bytes) return false;
   s.FS = FS;
   s.WX = WX;
   s.S = (uint32_t *)base;
   s.A = (uint64_t *)(base + align8(4ull * p));
-  s.XR = s.A + (uint64_t)p * W;
+  s.XR = (uint64_t *)xbase;
   s.XL = nullptr;
   s.XA = nullptr;
   s.XV = nullptr;
   return true;
 }
 
-__device__ __forceinline__ uint64_t bits_need(uint32_t p, uint32_t q, uint32_t lds_bytes, uint32_t stack_lds = 0) {
+// LDS bytes a bitmap subproblem asks the pool for: the whole layout plus
+// `levels` frames.
+__device__ __forceinline__ uint32_t bits_want(uint32_t p, uint32_t q, uint32_t W, uint32_t levels) {
+  const uint32_t WX = (q + 63) / 64;
+  return (uint32_t)min<uint64_t>(bits_fixed(p, q, W) + (uint64_t)bits_fs(W, WX) * min(p, levels), 0xFFFFFFFFull);
+}
+
+// ... with the outside rows left in HBM.
+__device__ __forceinline__ uint32_t bits_want_inner(uint32_t p, uint32_t q, uint32_t W, uint32_t levels) {
+  const uint32_t WX = (q + 63) / 64;
+  return (uint32_t)min<uint64_t>(align8(4ull * p) + 8ull * p * W + (uint64_t)bits_fs(W, WX) * min(p, levels),
+                                 0xFFFFFFFFull);
+}
+
+// HBM arena bytes a bitmap subproblem needs when the pool gives it no LDS.
+__device__ __forceinline__ uint64_t bits_need(uint32_t p, uint32_t q) {
   const uint32_t W = (p + 63) / 64, WX = (q + 63) / 64;
-  const uint64_t fixed = bits_fixed(p, q, W);
-  const uint32_t FS = bits_fs(W, WX);
-  if (fixed + 4ull * FS <= lds_bytes) {
-    uint32_t lv = (uint32_t)((lds_bytes - fixed) / FS);
-    if (stack_lds && lv > stack_lds) lv = stack_lds;
-    return p > lv ? (uint64_t)(p - lv) * FS : 0;
-  }
-  uint32_t lv = lds_bytes / FS;
-  if (stack_lds && lv > stack_lds) lv = stack_lds;
-  return fixed + (p > lv ? (uint64_t)(p - lv) * FS : 0);
+  return bits_fixed(p, q, W) + (uint64_t)p * bits_fs(W, WX);
 }
 
 // Builds A over S and the outside-X rows of the q vertices in `ids`; returns
@@ -798,53 +956,58 @@ __device__ __forceinline__ uint64_t build_bits(const BkParams &pr, Sub &s, const
   return E0;
 }
 
-// Pivot over P|X (index order) then the outside rows in E (row order).
+// Pivot over P|X (index order) then the outside rows in E (row order): the
+// largest |P & N(u)|, ties to the lowest index u, outside row e indexed p + e.
+// Each lane scores the candidates of its bit position in every nonzero word;
+// one reduction over key = (score + 1) << 13 | (8191 - index) picks the
+// winner (p <= 4096 and q <= 4096 keep index < 8192).
+template <bool kLA, bool kLX>
 __device__ __forceinline__ void pick_pivot_bits(const Sub &s, uint64_t Pw, uint64_t Xw, uint64_t Ew, uint32_t &piv,
                                                 uint32_t &best) {
   const uint32_t L = lane_id(), W = s.W;
-  uint32_t bs = 0, bk = kNone;
   const uint64_t C = Pw | Xw;
-  for (uint32_t k = 0; k < W; ++k) {
-    const uint64_t ck = rdlane64(C, k);
-    if (ck == 0) continue;
-    if ((ck >> L) & 1ull) {
+  const uint64_t pm = ballot(Pw != 0);
+  uint32_t key = 0;
+  for (uint64_t cm = ballot(C != 0); cm; cm &= cm - 1) {
+    const uint32_t k = ctz64(cm);
+    if ((rdlane64(C, k) >> L) & 1ull) {
       const uint32_t u = k * 64 + L;
-      const uint64_t *row = s.A + (uint64_t)u * W;
+      const uint64_t r0 = (uint64_t)u * W;
       uint32_t sc = 0;
-      for (uint32_t j = 0; j < W; ++j) sc += popc64(rdlane64(Pw, j) & row[j]);
-      if (bk == kNone || sc > bs) {
-        bs = sc;
-        bk = u;
+      for (uint64_t m = pm; m; m &= m - 1) {
+        const uint32_t j = ctz64(m);
+        sc += popc64(rdlane64(Pw, j) & ld64<kLA>(s.A, r0 + j));
       }
+      key = max(key, ((sc + 1u) << 13) | (8191u - u));
     }
   }
-  for (uint32_t k = 0; k < s.WX; ++k) {
-    const uint64_t ek = rdlane64(Ew, k);
-    if (ek == 0) continue;
-    if ((ek >> L) & 1ull) {
+  for (uint64_t em = ballot(Ew != 0); em; em &= em - 1) {
+    const uint32_t k = ctz64(em);
+    if ((rdlane64(Ew, k) >> L) & 1ull) {
       const uint32_t e = k * 64 + L;
-      const uint64_t *row = s.XR + (uint64_t)e * W;
+      const uint64_t r0 = (uint64_t)e * W;
       uint32_t sc = 0;
-      for (uint32_t j = 0; j < W; ++j) sc += popc64(rdlane64(Pw, j) & row[j]);
-      if (bk == kNone || sc > bs) {
-        bs = sc;
-        bk = s.p + e;
+      for (uint64_t m = pm; m; m &= m - 1) {
+        const uint32_t j = ctz64(m);
+        sc += popc64(rdlane64(Pw, j) & ld64<kLX>(s.XR, r0 + j));
       }
+      key = max(key, ((sc + 1u) << 13) | (8191u - (s.p + e)));
     }
   }
-  const uint32_t mx = wave_max(bk == kNone ? 0u : bs + 1u);
-  piv = wave_min((bk != kNone && bs + 1u == mx) ? bk : kNone);
-  best = mx - 1u;
+  const uint32_t mx = wave_max(key);
+  piv = mx ? 8191u - (mx & 8191u) : kNone;
+  best = mx ? (mx >> 13) - 1u : 0u;
 }
 
 // Outside rows of E adjacent to S[c] (this lane's word of the result).
+template <bool kLX>
 __device__ __forceinline__ uint64_t filter_rows(const Sub &s, uint64_t Ew, uint32_t c) {
   const uint32_t L = lane_id(), W = s.W, cw = c >> 6, cb = c & 63;
   uint64_t out = 0;
-  for (uint32_t k = 0; k < s.WX; ++k) {
+  for (uint64_t em = ballot(Ew != 0); em; em &= em - 1) {
+    const uint32_t k = ctz64(em);
     const uint64_t ek = rdlane64(Ew, k);
-    if (ek == 0) continue;
-    const bool t = ((ek >> L) & 1ull) && ((s.XR[(uint64_t)(k * 64 + L) * W + cw] >> cb) & 1ull);
+    const bool t = ((ek >> L) & 1ull) && ((ld64<kLX>(s.XR, (uint64_t)(k * 64 + L) * W + cw) >> cb) & 1ull);
     const uint64_t m = ballot(t);
     if (L == k) out = m;
   }
@@ -852,48 +1015,88 @@ __device__ __forceinline__ uint64_t filter_rows(const Sub &s, uint64_t Ew, uint3
 }
 
 // Does any outside row of E touch S[c]?
+template <bool kLX>
 __device__ __forceinline__ bool rows_any(const Sub &s, uint64_t Ew, uint32_t c) {
   const uint32_t L = lane_id(), W = s.W, cw = c >> 6, cb = c & 63;
-  for (uint32_t k = 0; k < s.WX; ++k) {
+  for (uint64_t em = ballot(Ew != 0); em; em &= em - 1) {
+    const uint32_t k = ctz64(em);
     const uint64_t ek = rdlane64(Ew, k);
-    if (ek == 0) continue;
-    if (ballot(((ek >> L) & 1ull) && ((s.XR[(uint64_t)(k * 64 + L) * W + cw] >> cb) & 1ull))) return true;
+    if (ballot(((ek >> L) & 1ull) && ((ld64<kLX>(s.XR, (uint64_t)(k * 64 + L) * W + cw) >> cb) & 1ull))) return true;
   }
   return false;
 }
 
+// One frame of the bitmap solver: P, X, E words, then the pivot and R's hash.
+template <bool kL>
+__device__ __forceinline__ void frame_put(uint8_t *f, uint32_t W, uint32_t WX, uint64_t P, uint64_t X, uint64_t E,
+                                          uint32_t piv, uint64_t mix) {
+  const uint32_t L = lane_id();
+  uint64_t *fw = (uint64_t *)f;
+  if (L < W) {
+    st64<kL>(fw, L, P);
+    st64<kL>(fw, W + L, X);
+  }
+  if (L < WX) st64<kL>(fw, 2 * W + L, E);
+  if (L == 0) {
+    st32<kL>((uint32_t *)(f + 8 * (2 * W + WX)), 0, piv);
+    st64<kL>((uint64_t *)(f + 8 * (2 * W + WX) + 8), 0, mix);
+  }
+}
+
+template <bool kL>
+__device__ __forceinline__ void frame_get(const uint8_t *f, uint32_t W, uint32_t WX, uint64_t &P, uint64_t &X,
+                                          uint64_t &E, uint32_t &piv, uint64_t &mix) {
+  const uint32_t L = lane_id();
+  const uint64_t *fw = (const uint64_t *)f;
+  if (L < W) {
+    P = ld64<kL>(fw, L);
+    X = ld64<kL>(fw, W + L);
+  }
+  E = L < WX ? ld64<kL>(fw, 2 * W + L) : 0;
+  piv = uniform(ld32<kL>((const uint32_t *)(f + 8 * (2 * W + WX)), 0));
+  mix = uniform64(ld64<kL>((const uint64_t *)(f + 8 * (2 * W + WX) + 8), 0));
+}
+
+// This lane's word of the pivot's row: an inside vertex's row of A or an
+// outside row.
+template <bool kLA, bool kLX>
+__device__ __forceinline__ uint64_t pivot_word(const Sub &s, uint32_t piv) {
+  const uint32_t L = lane_id();
+  if (L >= s.W) return 0;
+  return piv < s.p ? ld64<kLA>(s.A, (uint64_t)piv * s.W + L) : ld64<kLX>(s.XR, (uint64_t)(piv - s.p) * s.W + L);
+}
+
 // Publishes the next child of the shallowest frame with a large enough child
 // (bitmap path): the donor's S, A and outside rows plus the child's P, X, E.
+template <bool kLA, bool kLX>
 __device__ __forceinline__ bool donate_bits(const BkParams &pr, Sub &s, uint32_t d, uint64_t &cP, uint64_t &cX, uint64_t cE,
                             uint32_t cpiv, uint64_t cmix, WaveAcc &wa) {
   const uint32_t L = lane_id(), W = s.W, WX = s.WX;
   const bool act = L < W, actx = L < WX;
-  if (s.p > pr.slot_pmax || s.q > pr.slot_xcap) return false;
+  // a receiver may get no LDS from its pool: it must fit the HBM arena alone
+  if (s.p > pr.slot_pmax || s.q > pr.slot_xcap || bits_need(s.p, s.q) > pr.hbm_bytes) return false;
   for (uint32_t j = 0; j <= d; ++j) {
     uint64_t Pj = cP, Xj = cX, Ej = cE, mixj = cmix;
     uint32_t pivj = cpiv;
     uint8_t *f = nullptr;
+    const bool fl = j < s.lds_levels;
     if (j < d) {
       f = frame_at(s, j);
-      const uint64_t *fw = (const uint64_t *)f;
-      Pj = act ? fw[L] : 0;
-      Xj = act ? fw[W + L] : 0;
-      Ej = actx ? fw[2 * W + L] : 0;
-      const uint8_t *meta = f + 8 * (2 * W + WX);
-      pivj = uniform(*(const uint32_t *)meta);
-      mixj = uniform64(*(const uint64_t *)(meta + 8));
+      Pj = Xj = 0;
+      if (fl) frame_get<true>(f, W, WX, Pj, Xj, Ej, pivj, mixj);
+      else frame_get<false>(f, W, WX, Pj, Xj, Ej, pivj, mixj);
     }
-    const uint64_t Nj = act ? row_of(s, pivj)[L] : 0;
+    const uint64_t Nj = pivot_word<kLA, kLX>(s, pivj);
     const uint64_t B = Pj & ~Nj;
     const uint64_t bm = ballot(B != 0);
     if (bm == 0) continue;
     const uint32_t k0 = ctz64(bm);
     const uint32_t bit = ctz64(rdlane64(B, k0));
     const uint32_t c = k0 * 64 + bit;
-    const uint64_t Ac = act ? s.A[(uint64_t)c * W + L] : 0;
+    const uint64_t Ac = act ? ld64<kLA>(s.A, (uint64_t)c * W + L) : 0;
     const uint64_t Pn = Pj & Ac, Xn = Xj & Ac;
     if (wave_sum(popc64(Pn)) < pr.donate_min_p) continue;
-    const uint64_t En = filter_rows(s, Ej, c);
+    const uint64_t En = filter_rows<kLX>(s, Ej, c);
     unsigned long long pos = 0;
     if (L == 0) {
       pos = ring_claim_push(pr);
@@ -935,8 +1138,13 @@ __device__ __forceinline__ bool donate_bits(const BkParams &pr, Sub &s, uint32_t
         cX |= m;
       }
     } else if (L == k0) {
-      ((uint64_t *)f)[L] = Pj & ~m;
-      ((uint64_t *)f)[W + L] = Xj | m;
+      if (fl) {
+        st64<true>((uint64_t *)f, L, Pj & ~m);
+        st64<true>((uint64_t *)f, W + L, Xj | m);
+      } else {
+        st64<false>((uint64_t *)f, L, Pj & ~m);
+        st64<false>((uint64_t *)f, W + L, Xj | m);
+      }
     }
     wa.nodes++;
     wa.donations++;
@@ -946,17 +1154,17 @@ __device__ __forceinline__ bool donate_bits(const BkParams &pr, Sub &s, uint32_t
 }
 
 // Runs BK from (P, X, E) on an arena that build_bits (or a received task)
-// has filled.
-template <bool kWL, int kStat>
-__device__ __forceinline__ void solve_bits(const BkParams &pr, Sub &s, uint64_t cP, uint64_t cX, uint64_t cE,
-                           unsigned long long *hist_lds, WaveAcc &wa, Clock<kStat> &clk) {
+// has filled; kLA / kLX: S and A / the outside rows are in LDS.
+template <bool kWL, int kStat, bool kLA, bool kLX>
+__device__ __forceinline__ void solve_bits_t(const BkParams &pr, Sub &s, uint64_t cP, uint64_t cX, uint64_t cE,
+                                             unsigned long long *hist_lds, WaveAcc &wa, Clock<kStat> &clk) {
   const uint32_t L = lane_id(), W = s.W, WX = s.WX;
-  const bool act = L < W, actx = L < WX;
+  const bool act = L < W;
   uint32_t piv, best;
   clk.to(kBdPivot);
-  pick_pivot_bits(s, cP, cX, cE, piv, best);
+  pick_pivot_bits<kLA, kLX>(s, cP, cX, cE, piv, best);
   if (piv == kNone || best == wave_sum(popc64(cP))) return;
-  uint64_t cN = act ? row_of(s, piv)[L] : 0;
+  uint64_t cN = pivot_word<kLA, kLX>(s, piv);
   uint64_t mix = s.rmix;
   uint32_t d = 0;
   while (true) {
@@ -964,11 +1172,12 @@ __device__ __forceinline__ void solve_bits(const BkParams &pr, Sub &s, uint64_t 
       if (wa.nodes - wa.last_poll >= pr.donate_check) {
         wa.last_poll = wa.nodes;
         clk.to(kBdQueue);
-        wa.budget = donation_budget(pr);
+        // only a task that has run for a while gives work away
+        wa.budget = __builtin_amdgcn_s_memrealtime() - wa.t_task >= pr.donate_age ? donation_budget(pr) : 0;
       }
       if (wa.budget > 0) {
         clk.to(kBdQueue);
-        if (donate_bits(pr, s, d, cP, cX, cE, piv, mix, wa)) wa.budget--;
+        if (donate_bits<kLA, kLX>(pr, s, d, cP, cX, cE, piv, mix, wa)) wa.budget--;
         else wa.budget = 0;
       }
     }
@@ -979,17 +1188,9 @@ __device__ __forceinline__ void solve_bits(const BkParams &pr, Sub &s, uint64_t 
       if (d == 0) break;
       clk.to(kBdBranch);
       --d;
-      const uint8_t *f = frame_at(s, d);
-      const uint64_t *fw = (const uint64_t *)f;
-      if (act) {
-        cP = fw[L];
-        cX = fw[W + L];
-      }
-      cE = actx ? fw[2 * W + L] : 0;
-      const uint8_t *meta = f + 8 * (2 * W + WX);
-      piv = uniform(*(const uint32_t *)meta);
-      mix = uniform64(*(const uint64_t *)(meta + 8));
-      cN = act ? row_of(s, piv)[L] : 0;
+      if (d < s.lds_levels) frame_get<true>(s.fr_lds + (uint64_t)d * s.FS, W, WX, cP, cX, cE, piv, mix);
+      else frame_get<false>(s.fr_hbm + (uint64_t)(d - s.lds_levels) * s.FS, W, WX, cP, cX, cE, piv, mix);
+      cN = pivot_word<kLA, kLX>(s, piv);
       continue;
     }
     const uint32_t k0 = ctz64(bm);
@@ -999,39 +1200,39 @@ __device__ __forceinline__ void solve_bits(const BkParams &pr, Sub &s, uint64_t 
       cP &= ~(1ull << bit);
       cX |= 1ull << bit;
     }
-    const uint64_t Ac = act ? s.A[(uint64_t)c * W + L] : 0;
+    const uint64_t Ac = act ? ld64<kLA>(s.A, (uint64_t)c * W + L) : 0;
     const uint64_t Pn = cP & Ac, Xn = cX & Ac;
     wa.nodes++;
-    const uint64_t cmix = mix + splitmix64(uniform(s.S[c]));
+    const uint64_t cmix = mix + splitmix64(uniform(ld32<kLA>(s.S, c)));
     if (ballot(Pn != 0) == 0) {
       clk.to(kBdMaximal);
-      if (ballot(Xn != 0) == 0 && !rows_any(s, cE, c)) report(pr, hist_lds, wa, s.rsize + d + 1, cmix);
+      if (ballot(Xn != 0) == 0 && !rows_any<kLX>(s, cE, c)) report(pr, hist_lds, wa, s.rsize + d + 1, cmix);
       continue;
     }
-    const uint64_t En = filter_rows(s, cE, c);
+    const uint64_t En = filter_rows<kLX>(s, cE, c);
     uint32_t npiv, nbest;
     clk.to(kBdPivot);
-    pick_pivot_bits(s, Pn, Xn, En, npiv, nbest);
+    pick_pivot_bits<kLA, kLX>(s, Pn, Xn, En, npiv, nbest);
     if (npiv == kNone || nbest == wave_sum(popc64(Pn))) continue;
     clk.to(kBdBranch);
-    uint8_t *f = frame_at(s, d);
-    uint64_t *fw = (uint64_t *)f;
-    if (act) {
-      fw[L] = cP;
-      fw[W + L] = cX;
-    }
-    if (actx) fw[2 * W + L] = cE;
-    uint8_t *meta = f + 8 * (2 * W + WX);
-    *(uint32_t *)meta = piv;
-    *(uint64_t *)(meta + 8) = mix;
+    if (d < s.lds_levels) frame_put<true>(s.fr_lds + (uint64_t)d * s.FS, W, WX, cP, cX, cE, piv, mix);
+    else frame_put<false>(s.fr_hbm + (uint64_t)(d - s.lds_levels) * s.FS, W, WX, cP, cX, cE, piv, mix);
     ++d;
     cP = Pn;
     cX = Xn;
     cE = En;
     piv = npiv;
     mix = cmix;
-    cN = act ? row_of(s, piv)[L] : 0;
+    cN = pivot_word<kLA, kLX>(s, piv);
   }
+}
+
+template <bool kWL, int kStat>
+__device__ __forceinline__ void solve_bits(const BkParams &pr, Sub &s, uint64_t cP, uint64_t cX, uint64_t cE,
+                                           unsigned long long *hist_lds, WaveAcc &wa, Clock<kStat> &clk) {
+  if (s.where == 3) solve_bits_t<kWL, kStat, true, true>(pr, s, cP, cX, cE, hist_lds, wa, clk);
+  else if (s.where == 1) solve_bits_t<kWL, kStat, true, false>(pr, s, cP, cX, cE, hist_lds, wa, clk);
+  else solve_bits_t<kWL, kStat, false, false>(pr, s, cP, cX, cE, hist_lds, wa, clk);
 }
 
 // Intersection of ascending a[0,na) and b[0,nb) written to out (ascending);
@@ -1125,13 +1326,24 @@ __device__ __forceinline__ void finish_bits(const BkParams &pr, Sub &s, const ui
 }
 
 template <bool kIPX, bool kWL, int kStat>
-__global__ __launch_bounds__(256) void bk_kernel(BkParams pr) {
+__global__ __launch_bounds__(kMaxWpg * kWave) void bk_kernel(BkParams pr) {
   extern __shared__ __align__(16) uint8_t smem[];
   unsigned long long *hist_lds = (unsigned long long *)smem;
   const uint32_t wig = threadIdx.x / kWave, L = lane_id();
+  const uint32_t hb = (uint32_t)align8(8ull * pr.hist_lds);
+  Pool pl;
+  pl.mask = (uint64_t *)(smem + hb);
+  pl.lock = (uint32_t *)(smem + hb + 8 * kPoolWords);
+  pl.base = smem + hb + kPoolHeader;
+  pl.ngran = pr.pool_bytes / kGranule;
   for (uint32_t i = threadIdx.x; i < pr.hist_lds; i += blockDim.x) hist_lds[i] = 0;
+  if (threadIdx.x < kPoolWords) {
+    const uint32_t lo = threadIdx.x * 64;
+    pl.mask[threadIdx.x] = lo >= pl.ngran ? ~0ull : (lo + 64 <= pl.ngran ? 0ull : (~0ull << (pl.ngran - lo)));
+  }
+  if (threadIdx.x == 0) *pl.lock = 0;
   __syncthreads();
-  uint8_t *lds = smem + align8(8ull * pr.hist_lds) + (uint64_t)wig * pr.lds_bytes;
+  const uint32_t lv = pr.want_levels;
   const uint32_t gw = blockIdx.x * pr.wpg + wig;
   uint8_t *hbm = pr.hbm + (uint64_t)gw * pr.hbm_bytes;
   WaveAcc wa;
@@ -1162,6 +1374,7 @@ __global__ __launch_bounds__(256) void bk_kernel(BkParams pr) {
     clk.to(kBdInduced);
     for (uint64_t it = lo; it < hi; ++it) {
       wa.sub_cliques = 0;
+      wa.t_task = __builtin_amdgcn_s_memrealtime();
       if (pr.level == 1) {
         const uint32_t v = uniform(items1[it]);
         const uint64_t b0 = pr.rp[v], s0 = pr.split[v], e0 = pr.rp[v + 1];
@@ -1169,8 +1382,12 @@ __global__ __launch_bounds__(256) void bk_kernel(BkParams pr) {
         if (p == 0) continue;
         const uint32_t W = (p + 63) / 64;
         const bool bits = kIPX && q <= kMaxXRows;
-        if (!(bits ? place_bits(s, p, q, W, lds, pr.lds_bytes, hbm, pr.hbm_bytes, pr.stack_lds)
-                   : place(s, p, q, W, kIPX, lds, pr.lds_bytes, hbm, pr.hbm_bytes, pr.stack_lds))) {
+        const Region rg = bits ? pool_take(pl, bits_want(p, q, W, lv), bits_want_inner(p, q, W, lv),
+                                           bits_fs(W, (q + 63) / 64) * min(p, lv))
+                               : pool_take(pl, list_want(p, q, W, kIPX, lv), 0, (16 * W + 16) * min(p, lv));
+        if (!(bits ? place_bits(s, p, q, W, rg.ptr, rg.bytes, hbm, pr.hbm_bytes, pr.stack_lds)
+                   : place(s, p, q, W, kIPX, rg.ptr, rg.bytes, hbm, pr.hbm_bytes, pr.stack_lds))) {
+          pool_put(pl, rg);
           defer(pr, it);
           continue;
         }
@@ -1187,6 +1404,7 @@ __global__ __launch_bounds__(256) void bk_kernel(BkParams pr) {
           for (uint32_t i = L; i < q; i += kWave) s.XL[i] = pr.oc[b0 + i];
           finish_sub<kIPX, kWL, kStat>(pr, s, full_word(p), 0, hist_lds, wa, clk);
         }
+        pool_put(pl, rg);
       } else {
         const uint2 a = items2[it];
         const uint32_t u = uniform(a.x), w = uniform(a.y);
@@ -1194,7 +1412,10 @@ __global__ __launch_bounds__(256) void bk_kernel(BkParams pr) {
         const uint64_t sw = pr.split[w], ew = pr.rp[w + 1], bw = pr.rp[w];
         const uint32_t pb = (uint32_t)min(eu - su, ew - sw), qb = (uint32_t)min(eu - bu, sw - bw);
         Sub t = s;
-        if (!place(t, pb, qb, (pb + 63) / 64, kIPX, lds, pr.lds_bytes, hbm, pr.hbm_bytes, pr.stack_lds)) {
+        const uint32_t Wb = (pb + 63) / 64;
+        const Region rg = pool_take(pl, list_want(pb, qb, Wb, kIPX, lv), 0, (16 * Wb + 16) * min(pb, lv));
+        if (!place(t, pb, qb, Wb, kIPX, rg.ptr, rg.bytes, hbm, pr.hbm_bytes, pr.stack_lds)) {
+          pool_put(pl, rg);
           defer(pr, it);
           continue;
         }
@@ -1204,13 +1425,15 @@ __global__ __launch_bounds__(256) void bk_kernel(BkParams pr) {
         if (p == 0) {
           if (q == 0) report(pr, hist_lds, wa, 2, rmix);
           if (pr.sub_cliques && L == 0 && wa.sub_cliques) atomicAdd(&pr.sub_cliques[u], wa.sub_cliques);
+          pool_put(pl, rg);
           continue;
         }
         // re-lay the arena for the actual sizes
         const uint32_t W = (p + 63) / 64;
         const bool bits = kIPX && q <= kMaxXRows;
-        if (!(bits ? place_bits(s, p, q, W, lds, pr.lds_bytes, hbm, pr.hbm_bytes, pr.stack_lds)
-                   : place(s, p, q, W, kIPX, lds, pr.lds_bytes, hbm, pr.hbm_bytes, pr.stack_lds))) {
+        if (!(bits ? place_bits(s, p, q, W, rg.ptr, rg.bytes, hbm, pr.hbm_bytes, pr.stack_lds)
+                   : place(s, p, q, W, kIPX, rg.ptr, rg.bytes, hbm, pr.hbm_bytes, pr.stack_lds))) {
+          pool_put(pl, rg);
           defer(pr, it);
           continue;
         }
@@ -1226,6 +1449,7 @@ __global__ __launch_bounds__(256) void bk_kernel(BkParams pr) {
         s.rmix = rmix;
         if (bits) finish_bits<kWL, kStat>(pr, s, t.XL, hist_lds, wa, clk);
         else finish_sub<kIPX, kWL, kStat>(pr, s, full_word(p), 0, hist_lds, wa, clk);
+        pool_put(pl, rg);
       }
     }
     if (kWL && L == 0) atomicSub(pr.ctl + kCtlOutstanding, 1ull);
@@ -1251,8 +1475,11 @@ __global__ __launch_bounds__(256) void bk_kernel(BkParams pr) {
                        nx = uniform(h[4]), kind = uniform(h[5]);
         const uint64_t rmix = uniform64(*(const uint64_t *)(slot + 24));
         if (kIPX && kind == kTaskBits) {
-          const bool okb = p <= pr.slot_pmax && W == (p + 63) / 64 && nx <= pr.slot_xcap && nx <= kMaxXRows &&
-                           place_bits(s, p, nx, W, lds, pr.lds_bytes, hbm, pr.hbm_bytes, pr.stack_lds);
+          const bool shape = p <= pr.slot_pmax && W == (p + 63) / 64 && nx <= pr.slot_xcap && nx <= kMaxXRows;
+          const Region rg = shape ? pool_take(pl, bits_want(p, nx, W, lv), bits_want_inner(p, nx, W, lv),
+                                              bits_fs(W, (nx + 63) / 64) * min(p, lv))
+                                  : Region{pl.base, 0, 0, 0};
+          const bool okb = shape && place_bits(s, p, nx, W, rg.ptr, rg.bytes, hbm, pr.hbm_bytes, pr.stack_lds);
           uint64_t P0 = 0, X0 = 0, E0 = 0;
           if (okb) {
             const uint64_t *sp = (const uint64_t *)(slot + pr.slot_oP);
@@ -1282,6 +1509,7 @@ __global__ __launch_bounds__(256) void bk_kernel(BkParams pr) {
             s.root = root;
             s.rmix = rmix;
             wa.sub_cliques = 0;
+            wa.t_task = __builtin_amdgcn_s_memrealtime();
             const uint64_t nodes0 = wa.nodes;
             solve_bits<kWL, kStat>(pr, s, P0, X0, E0, hist_lds, wa, clk);
             clk.to(kBdOther);
@@ -1293,12 +1521,15 @@ __global__ __launch_bounds__(256) void bk_kernel(BkParams pr) {
           } else if (L == 0) {
             atomicOr(&pr.acc[kAccError], 2ull);
           }
+          pool_put(pl, rg);
           if (L == 0) atomicSub(pr.ctl + kCtlOutstanding, 1ull);
           t0 = __builtin_amdgcn_s_memrealtime();
           continue;
         }
-        const bool ok = kind == kTaskList && p <= pr.slot_pmax && W == (p + 63) / 64 && nx <= pr.slot_xcap &&
-                        place(s, p, nx, W, kIPX, lds, pr.lds_bytes, hbm, pr.hbm_bytes, pr.stack_lds);
+        const bool shape = kind == kTaskList && p <= pr.slot_pmax && W == (p + 63) / 64 && nx <= pr.slot_xcap;
+        const Region rg = shape ? pool_take(pl, list_want(p, nx, W, kIPX, lv), 0, (16 * W + 16) * min(p, lv))
+                                : Region{pl.base, 0, 0, 0};
+        const bool ok = shape && place(s, p, nx, W, kIPX, rg.ptr, rg.bytes, hbm, pr.hbm_bytes, pr.stack_lds);
         uint64_t P0 = 0, X0 = 0;
         if (ok) {
           const uint64_t *sp = (const uint64_t *)(slot + pr.slot_oP);
@@ -1335,6 +1566,7 @@ __global__ __launch_bounds__(256) void bk_kernel(BkParams pr) {
           s.root = root;
           s.rmix = rmix;
           wa.sub_cliques = 0;
+          wa.t_task = __builtin_amdgcn_s_memrealtime();
           const uint64_t nodes0 = wa.nodes;
           bk_solve<kIPX, kWL, kStat>(pr, s, P0, X0, hist_lds, wa, clk);
           clk.to(kBdOther);
@@ -1345,6 +1577,7 @@ __global__ __launch_bounds__(256) void bk_kernel(BkParams pr) {
         } else if (L == 0) {
           atomicOr(&pr.acc[kAccError], 2ull);  // donors only publish what fits
         }
+        pool_put(pl, rg);
         if (L == 0) atomicSub(pr.ctl + kCtlOutstanding, 1ull);
         t0 = __builtin_amdgcn_s_memrealtime();
         continue;
@@ -1357,9 +1590,9 @@ __global__ __launch_bounds__(256) void bk_kernel(BkParams pr) {
         if (L == 0) atomicOr(&pr.acc[kAccError], 1ull);
         break;
       }
-      // back off up to ~16 x 2K cycles between polls of the shared ring lines
+      // back off up to ~64 x 2K cycles between polls of the shared ring lines
       for (uint32_t z = 0; z < backoff; ++z) __builtin_amdgcn_s_sleep(32);
-      backoff = min(backoff * 2, 16u);
+      backoff = min(backoff * 2, 64u);
     }
   }
   clk.to(kBdOther);
@@ -1396,8 +1629,7 @@ __global__ void ring_init(unsigned long long *seq, uint32_t cap, unsigned long l
 
 // Largest HBM arena any item needs, and how many need more than `cap`.
 __global__ void need_kernel(const uint64_t *rp, const uint64_t *split, const void *items, uint64_t n_items,
-                            uint32_t level, bool ipx, uint32_t lds_bytes, uint32_t stack_lds, uint64_t cap,
-                            unsigned long long *out) {
+                            uint32_t level, bool ipx, uint64_t cap, unsigned long long *out) {
   unsigned long long mx = 0, over = 0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_items; i += stride) {
@@ -1414,12 +1646,12 @@ __global__ void need_kernel(const uint64_t *rp, const uint64_t *split, const voi
     // level-2 items stage their intersections in the list layout of the
     // bounds, then re-lay for the actual sizes (up to four more HBM frames)
     uint64_t need;
+    // (sizes for a subproblem the LDS pool gives nothing)
     if (level == 2)
-      need = max(hbm_need(p, q, ipx, lds_bytes, stack_lds),
-                 ipx && q <= kMaxXRows ? bits_need(p, q, lds_bytes, stack_lds) : 0ull) +
+      need = max(hbm_need(p, q, ipx), ipx && q <= kMaxXRows ? bits_need(p, q) : 0ull) +
              4ull * (8 * (2 * ((p + 63) / 64) + (q + 63) / 64) + 16);
     else
-      need = ipx && q <= kMaxXRows ? bits_need(p, q, lds_bytes, stack_lds) : hbm_need(p, q, ipx, lds_bytes, stack_lds);
+      need = ipx && q <= kMaxXRows ? bits_need(p, q) : hbm_need(p, q, ipx);
     mx = max(mx, (unsigned long long)need);
     over += need > cap;
   }
@@ -1496,23 +1728,28 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   hipDeviceProp_t prop;
   MCE_HIP(hipGetDeviceProperties(&prop, dev));
   const int stat = stat_of(cfg.task);
-  const uint32_t wpg = std::max<uint32_t>(1, std::min<uint32_t>(cfg.waves_per_group ? cfg.waves_per_group : 4, 4));
   const uint32_t hist_lds = std::min<uint32_t>(hist_len, 128);
   const size_t hist_bytes = (8ull * hist_lds + 7) & ~7ull;
   const size_t lds_cu = 160 * 1024;
-  uint32_t lds_wave = cfg.lds_bytes;
-  if (!lds_wave) {
-    // fill the CU's LDS at the occupancy the registers allow
+  // Waves per workgroup: by default all the waves the registers let one CU
+  // hold, in as few workgroups as the launch bound allows, so that one LDS
+  // pool serves the whole CU.
+  uint32_t wpg = cfg.waves_per_group;
+  uint32_t groups_cu = std::max<uint32_t>(cfg.groups_per_cu, 1);
+  if (!wpg) {
     int nb_regs = 0;
     BkParams probe{};
-    int rc0 = dispatch(cfg.ipx, cfg.wl, stat, true, &nb_regs, probe, 0, wpg, hist_bytes + wpg * 1024, s);
+    int rc0 = dispatch(cfg.ipx, cfg.wl, stat, true, &nb_regs, probe, 0, 4, hist_bytes + kPoolHeader + 4096, s);
     if (rc0) return rc0;
-    nb_regs = std::max(nb_regs, 1);
-    const size_t per_group = lds_cu / (size_t)nb_regs;
-    lds_wave = (uint32_t)std::min<size_t>(48 * 1024, ((per_group - hist_bytes) / wpg) & ~63ull);
-    lds_wave = std::max<uint32_t>(lds_wave, 2048);
+    const uint32_t waves_cu = 4 * (uint32_t)std::max(nb_regs, 1);
+    groups_cu = std::max<uint32_t>(groups_cu, (waves_cu + kMaxWpg - 1) / kMaxWpg);
+    wpg = std::max<uint32_t>(1, waves_cu / groups_cu);
   }
-  const size_t lds = hist_bytes + (size_t)wpg * lds_wave;
+  wpg = std::max<uint32_t>(1, std::min<uint32_t>(wpg, kMaxWpg));
+  size_t pool = cfg.lds_bytes ? (size_t)cfg.lds_bytes * wpg
+                              : lds_cu / groups_cu - hist_bytes - kPoolHeader;
+  pool = std::min<size_t>(pool, (size_t)kPoolWords * 64 * kGranule) / kGranule * kGranule;
+  const size_t lds = hist_bytes + kPoolHeader + pool;
   if (lds > lds_cu) return set_error(MCE_EINVAL, "LDS per workgroup %zu exceeds 160 KiB", lds);
 
   // every device buffer of this call, released on all paths
@@ -1538,8 +1775,7 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   unsigned long long *need = acc + kAccN;
   MCE_HIP(hipMemsetAsync(acc, 0, sizeof(unsigned long long) * (kAccN + 2), s));
   const uint64_t cap = 16ull << 20;
-  need_kernel<<<1024, 256, 0, s>>>(g.rp, g.split, d_items, n_items, (uint32_t)cfg.level, cfg.ipx, lds_wave,
-                                   cfg.stack_lds_levels, cap, need);
+  need_kernel<<<1024, 256, 0, s>>>(g.rp, g.split, d_items, n_items, (uint32_t)cfg.level, cfg.ipx, cap, need);
   unsigned long long hneed[2];
   MCE_HIP(hipMemcpyAsync(hneed, need, sizeof(hneed), hipMemcpyDeviceToHost, s));
   MCE_HIP(hipStreamSynchronize(s));
@@ -1548,8 +1784,8 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   BkParams pr{};
   if ((rc = dispatch(cfg.ipx, cfg.wl, stat, true, &nb, pr, 0, wpg, lds, s))) return rc;
   nb = std::max(nb, 1);
-  uint32_t groups = cfg.groups ? cfg.groups : (uint32_t)(nb * prop.multiProcessorCount);
-  groups = std::min<uint32_t>(groups, (uint32_t)(nb * prop.multiProcessorCount));  // all waves co-resident
+  if (cfg.groups_per_cu) nb = std::min<int>(nb, (int)cfg.groups_per_cu);
+  uint32_t groups = (uint32_t)(nb * prop.multiProcessorCount);  // all waves co-resident
   uint64_t arena = std::min<uint64_t>(hneed[0], cap);
   arena = (arena + 255) & ~255ull;
   size_t freeb = 0, totalb = 0;
@@ -1580,8 +1816,9 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   pr.t1 = std::min<uint64_t>(n_items, (uint64_t)waves * 8);
   pr.chunk = 8;
   pr.level = (uint32_t)cfg.level;
-  pr.lds_bytes = lds_wave;
+  pr.pool_bytes = (uint32_t)pool;
   pr.stack_lds = cfg.stack_lds_levels;
+  pr.want_levels = cfg.stack_lds_levels ? cfg.stack_lds_levels : 24;
   pr.hbm = hbm;
   pr.hbm_bytes = arena;
   pr.acc = acc;
@@ -1618,6 +1855,7 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
     pr.ring_cap = waves;
     pr.donate_min_p = std::max<uint32_t>(cfg.donate_min_p, 1);
     pr.donate_check = std::max<uint32_t>(cfg.donate_check_nodes, 1);
+    pr.donate_age = (uint64_t)cfg.donate_age_us * 100;
     if ((rc = dmalloc((void **)&pr.ctl, sizeof(unsigned long long) * kCtlWords))) return rc;
     if ((rc = dmalloc((void **)&pr.seq, sizeof(unsigned long long) * pr.ring_cap))) return rc;
     if ((rc = dmalloc((void **)&pr.ring, pr.slot_bytes * pr.ring_cap))) return rc;

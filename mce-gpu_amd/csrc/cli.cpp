@@ -41,12 +41,13 @@ void usage() {
           "\n    -h                            Help"
           "\n"
           "\nBlock settings (0 = automatic):"
-          "\n    --waves-per-group N   wave64s per workgroup (1..4)"
+          "\n    --waves-per-group N   wave64s per workgroup (1..12), sharing one LDS pool"
           "\n    --groups-per-cu N     resident workgroups per CU"
-          "\n    --lds-arena BYTES     LDS arena per wave (bitmap cutoff for LDS-resident subproblems)"
+          "\n    --lds-arena BYTES     LDS pool bytes per wave (bitmap cutoff for LDS-resident subproblems)"
           "\n    --stack-lds N         BK frames kept in LDS before spilling to HBM"
           "\n    --donate-min-p N      donate a child branch only if |P| >= N (default 10)"
           "\n    --donate-check N      tree nodes between idle-worker polls"
+          "\n    --donate-age US       a task donates only after running this long"
           "\n    --schedule plain|learned   first-level order (default plain)"
           "\n    --model PATH          cost model for --schedule learned"
           "\n"
@@ -96,6 +97,7 @@ int main(int argc, char **argv) {
                                      {"donate-check", required_argument, 0, 6},
                                      {"schedule", required_argument, 0, 7},
                                      {"model", required_argument, 0, 8},
+                                     {"donate-age", required_argument, 0, 9},
                                      {0, 0, 0, 0}};
   int opt;
   while ((opt = getopt_long(argc, argv, "g:r:d:m:p:i:w:h", longopts, nullptr)) >= 0) {
@@ -152,6 +154,7 @@ int main(int argc, char **argv) {
         else fail0("Unrecognized --schedule option (plain or learned)");
         break;
       case 8: model = optarg; break;
+      case 9: opts.donate_age_us = atoi(optarg); break;
       default:
         fprintf(stderr, "\nUnrecognized option!\n");
         usage();

@@ -47,13 +47,14 @@ struct BkLaunch {
   bool ipx = true;
   bool wl = false;
   int task = 0;        // enum mce_task
-  uint32_t waves_per_group = 4;
-  uint32_t groups = 0; // 0 = fill the device
-  uint32_t lds_bytes = 0;
+  uint32_t waves_per_group = 0;  // 0 = the waves the registers let a CU hold, in few groups
+  uint32_t groups_per_cu = 0;    // 0 = as the waves per group imply
+  uint32_t lds_bytes = 0;        // LDS pool bytes per wave (0 = the CU's LDS split by groups)
   uint32_t stack_lds_levels = 0;
   uint64_t hbm_bytes = 0;
   uint32_t donate_min_p = 10;
   uint32_t donate_check_nodes = 64;
+  uint32_t donate_age_us = 1000;
   bool capture = false;  // run the enumeration launch as a captured hipGraph
 };
 

@@ -78,7 +78,7 @@ def test_autotune_is_result_invariant(mce, oracle):
         best, trials = dg.autotune(sample_fraction=0.25, worklist=2)
         assert len(trials) >= 8 and all(t["ms"] >= 0 for t in trials)
         fastest = min(trials, key=lambda t: t["ms"])
-        for k in ("lds_arena_bytes", "waves_per_group", "stack_lds_levels"):
+        for k in ("waves_per_group", "groups_per_cu", "stack_lds_levels"):
             assert best[k] == fastest[k]
         r = dg.count(level=1, induced=2, worklist=2, **best)
         assert (r["count"], r["hash"], r["hist"]) == (ref["count"], ref["hash"], ref["hist"])

@@ -7,7 +7,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 # donate at every poll, down to tiny subtrees, so donation really happens
-EAGER = dict(donate_min_p=2, donate_check_nodes=1)
+EAGER = dict(donate_min_p=2, donate_check_nodes=1, donate_age_us=1)
 
 
 def rand_graph(mce, n, p, seed):
