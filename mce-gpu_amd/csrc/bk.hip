@@ -67,7 +67,9 @@ enum Stat : int { kStatNone = 0, kStatLb = 1, kStatDonor = 2, kStatBd = 3 };
 enum Bd : int { kBdOther = 0, kBdInduced, kBdQueue, kBdPivot, kBdSetOps, kBdMaximal, kBdBranch };
 
 // Ring control words, each on its own 64-byte line.
-constexpr uint32_t kCtlHead = 0, kCtlTail = 8, kCtlOutstanding = 16, kCtlIdle = 24, kCtlWords = 32;
+constexpr uint32_t kCtlHead = 0, kCtlTail = 8, kCtlOutstanding = 16, kCtlWords = 32;
+// Mailbox words of the waves, each on its own 128-byte line.
+constexpr uint32_t kMboxStride = 16;
 constexpr unsigned long long kNoSlot = ~0ull;
 constexpr uint32_t kSlotHeader = 64;  // bytes
 // IPX subproblems with at most this many outside-X rows keep them as a bitmap
@@ -112,8 +114,10 @@ struct BkParams {
   unsigned long long *ctl;
   unsigned long long *seq;
   uint8_t *ring;
-  uint32_t ring_cap;
-  uint64_t slot_bytes;
+  uint32_t ring_cap;       // idle-queue capacity (= waves)
+  uint32_t *idle_ids;      // idle queue entries: wave ids
+  unsigned long long *mbox;  // per-wave mailbox: 1 once a task is in the wave's slot
+  uint64_t slot_bytes;       // task slot per wave (pr.ring + wave * slot_bytes)
   uint32_t slot_pmax;  // largest donor |S|
   uint32_t slot_wmax;
   uint32_t slot_xcap;  // largest outside-X list carried
@@ -176,7 +180,38 @@ struct Clock {
   }
 };
 
-__device__ __forceinline__ uint64_t align8(uint64_t x) { return (x + 7) & ~7ull; }
+__host__ __device__ __forceinline__ uint64_t align8(uint64_t x) { return (x + 7) & ~7ull; }
+__host__ __device__ __forceinline__ uint64_t min64(uint64_t a, uint64_t b) { return a < b ? a : b; }
+
+// All-ones P over the first p bits (lane k holds word k).
+__device__ __forceinline__ uint64_t full_word(uint32_t p) {
+  const uint32_t L = lane_id(), W = (p + 63) / 64;
+  if (L >= W) return 0;
+  return (L == W - 1 && (p & 63)) ? ((1ull << (p & 63)) - 1) : ~0ull;
+}
+
+// Forward wave copies, four loads in flight per lane; each chunk of 256
+// elements is read before it is written, so a destination below an
+// overlapping source is safe.
+template <class T>
+__device__ __forceinline__ void wave_copy_t(T *dst, const T *src, uint64_t n) {
+  const uint32_t L = lane_id();
+  for (uint64_t b = 0; b < n; b += 4 * kWave) {
+    const uint64_t i0 = b + L, i1 = i0 + kWave, i2 = i1 + kWave, i3 = i2 + kWave;
+    const T x0 = i0 < n ? src[i0] : T(0), x1 = i1 < n ? src[i1] : T(0);
+    const T x2 = i2 < n ? src[i2] : T(0), x3 = i3 < n ? src[i3] : T(0);
+    if (i0 < n) dst[i0] = x0;
+    if (i1 < n) dst[i1] = x1;
+    if (i2 < n) dst[i2] = x2;
+    if (i3 < n) dst[i3] = x3;
+  }
+}
+__device__ __forceinline__ void wave_copy(uint32_t *dst, const uint32_t *src, uint32_t n) {
+  wave_copy_t<uint32_t>(dst, src, n);
+}
+__device__ __forceinline__ void copy64(uint64_t *dst, const uint64_t *src, uint64_t n) {
+  wave_copy_t<uint64_t>(dst, src, n);
+}
 
 // Typed arena accesses: kL picks LDS (ds_*) over global memory (global_*).
 // Arena pointers are generic because a subproblem's placement is chosen at
@@ -214,7 +249,7 @@ __device__ __forceinline__ uint8_t *frame_at(const Sub &s, uint32_t d) {
 }
 
 // Fixed arena bytes for bounds (p, q, W).
-__device__ __forceinline__ uint64_t fixed_bytes(uint32_t p, uint32_t q, uint32_t W, bool ipx) {
+__host__ __device__ __forceinline__ uint64_t fixed_bytes(uint32_t p, uint32_t q, uint32_t W, bool ipx) {
   return align8(4ull * p) + 8ull * p * W + (ipx ? 8ull * q * W + align8(4ull * q) : 0) + 2 * align8(4ull * q);
 }
 
@@ -266,8 +301,8 @@ __device__ __forceinline__ uint64_t hbm_need(uint32_t p, uint32_t q, bool ipx) {
 }
 
 // LDS bytes a list-path subproblem asks the pool for.
-__device__ __forceinline__ uint32_t list_want(uint32_t p, uint32_t q, uint32_t W, bool ipx, uint32_t levels) {
-  return (uint32_t)min<uint64_t>(fixed_bytes(p, q, W, ipx) + (uint64_t)(16 * W + 16) * min(p, levels), 0xFFFFFFFFull);
+__host__ __device__ __forceinline__ uint32_t list_want(uint32_t p, uint32_t q, uint32_t W, bool ipx, uint32_t levels) {
+  return (uint32_t)min64(fixed_bytes(p, q, W, ipx) + (uint64_t)(16 * W + 16) * min64(p, levels), 0xFFFFFFFFull);
 }
 
 // Workgroup LDS pool: granules of kGranule bytes handed to waves for the
@@ -371,6 +406,24 @@ __device__ __forceinline__ void pool_put(const Pool &pl, const Region &r) {
     pool_mark(pl, r.g0, r.gk, false);
     pool_unlock(pl.lock);
   }
+}
+
+// pool_take that waits (up to about a millisecond) for at least `need` bytes
+// when the pool could hold them: a subproblem run from HBM costs far more
+// than the wait, and the waves holding LDS never wait, so space comes free.
+__device__ __forceinline__ Region pool_take_wait(const Pool &pl, uint32_t want, uint32_t mid, uint32_t low,
+                                                 uint32_t need) {
+  Region r = pool_take(pl, want, mid, low);
+  if (r.bytes >= need || need > pl.ngran * kGranule) return r;
+  uint32_t backoff = 1;
+  for (uint32_t t = 0; t < 24; ++t) {
+    pool_put(pl, r);
+    for (uint32_t z = 0; z < backoff; ++z) __builtin_amdgcn_s_sleep(32);
+    backoff = min(backoff * 2, 32u);
+    r = pool_take(pl, want, mid, low);
+    if (r.bytes >= need) return r;
+  }
+  return r;
 }
 
 __device__ __forceinline__ const uint64_t *row_of(const Sub &s, uint32_t piv) {
@@ -570,7 +623,7 @@ __device__ __forceinline__ unsigned long long ld_relaxed(unsigned long long *p) 
 __device__ __forceinline__ unsigned long long ring_claim_push(const BkParams &pr) {
   unsigned long long *tail = pr.ctl + kCtlTail;
   unsigned long long pos = ld_relaxed(tail);
-  for (int it = 0; it < 64; ++it) {
+  for (int it = 0; it < 8; ++it) {
     const unsigned long long sq =
         __hip_atomic_load(&pr.seq[pos % pr.ring_cap], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     const long long dif = (long long)(sq - pos);
@@ -591,7 +644,7 @@ __device__ __forceinline__ unsigned long long ring_claim_push(const BkParams &pr
 __device__ __forceinline__ unsigned long long ring_claim_pop(const BkParams &pr) {
   unsigned long long *head = pr.ctl + kCtlHead;
   unsigned long long pos = ld_relaxed(head);
-  for (int it = 0; it < 64; ++it) {
+  for (int it = 0; it < 8; ++it) {
     const unsigned long long sq =
         __hip_atomic_load(&pr.seq[pos % pr.ring_cap], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     const long long dif = (long long)(sq - (pos + 1));
@@ -609,14 +662,51 @@ __device__ __forceinline__ unsigned long long ring_claim_pop(const BkParams &pr)
 }
 
 // Idle waves minus queued tasks, sampled by lane 0.
+// Idle waves waiting for work: the length of the idle queue.
 __device__ __forceinline__ int32_t donation_budget(const BkParams &pr) {
   int32_t b = 0;
   if (lane_id() == 0) {
-    const unsigned long long idle = ld_relaxed(pr.ctl + kCtlIdle);
     const unsigned long long h = ld_relaxed(pr.ctl + kCtlHead), t = ld_relaxed(pr.ctl + kCtlTail);
-    b = (int32_t)idle - (int32_t)(t > h ? t - h : 0);
+    b = (int32_t)(t > h ? t - h : 0);
   }
   return (int32_t)uniform((uint32_t)b);
+}
+
+// Takes an idle wave off the idle queue for a donation and counts the task as
+// outstanding; returns that wave's id (its slot receives the task) or kNone.
+__device__ __forceinline__ uint32_t claim_receiver(const BkParams &pr) {
+  uint32_t r = kNone;
+  if (lane_id() == 0) {
+    const unsigned long long pos = ring_claim_pop(pr);
+    if (pos != kNoSlot) {
+      r = __hip_atomic_load(&pr.idle_ids[pos % pr.ring_cap], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&pr.seq[pos % pr.ring_cap], pos + pr.ring_cap, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      atomicAdd(pr.ctl + kCtlOutstanding, 1ull);
+    }
+  }
+  return uniform(r);
+}
+
+// Hands the task written into receiver r's slot over to it.
+__device__ __forceinline__ void publish_task(const BkParams &pr, uint32_t r) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  if (lane_id() == 0)
+    __hip_atomic_store(&pr.mbox[(uint64_t)r * kMboxStride], 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Puts this idle wave on the idle queue (its mailbox cleared first); false
+// when the queue's tail was too contended this time.
+__device__ __forceinline__ bool register_idle(const BkParams &pr, uint32_t gw) {
+  unsigned long long pos = kNoSlot;
+  if (lane_id() == 0) {
+    __hip_atomic_store(&pr.mbox[(uint64_t)gw * kMboxStride], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    pos = ring_claim_push(pr);
+    if (pos != kNoSlot) {
+      __hip_atomic_store(&pr.idle_ids[pos % pr.ring_cap], gw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&pr.seq[pos % pr.ring_cap], pos + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  return uniform64(pos) != kNoSlot;
 }
 
 // Publishes the next child of the shallowest frame that still has one, when
@@ -668,14 +758,9 @@ __device__ __forceinline__ bool donate_shallowest(const BkParams &pr, Sub &s, ui
       nx += popc64(ballot(i < xej && xe_adj<kIPX>(s, s.XL[i], c, nb, nlen)));
     }
     if (nx > pr.slot_xcap || hbm_need(s.p, nx, kIPX) > pr.hbm_bytes) continue;
-    unsigned long long pos = 0;
-    if (L == 0) {
-      pos = ring_claim_push(pr);
-      if (pos != kNoSlot) atomicAdd(pr.ctl + kCtlOutstanding, 1ull);
-    }
-    pos = uniform64(pos);
-    if (pos == kNoSlot) return false;
-    uint8_t *slot = pr.ring + (pos % pr.ring_cap) * (uint64_t)pr.slot_bytes;
+    const uint32_t rcv = claim_receiver(pr);
+    if (rcv == kNone) return false;
+    uint8_t *slot = pr.ring + (uint64_t)rcv * pr.slot_bytes;
     if (L == 0) {
       uint32_t *h = (uint32_t *)slot;
       h[0] = s.root;
@@ -695,7 +780,7 @@ __device__ __forceinline__ bool donate_shallowest(const BkParams &pr, Sub &s, ui
     for (uint32_t i = L; i < s.p; i += kWave) sS[i] = s.S[i];
     uint64_t *sA = (uint64_t *)(slot + pr.slot_oA);
     const uint64_t nA = (uint64_t)s.p * W;
-    for (uint64_t i = L; i < nA; i += kWave) sA[i] = s.A[i];
+    copy64(sA, s.A, nA);
     uint32_t *sXI = (uint32_t *)(slot + pr.slot_oXI);
     uint64_t *sXR = (uint64_t *)(slot + pr.slot_oXR);
     uint32_t w = 0;
@@ -719,8 +804,7 @@ __device__ __forceinline__ bool donate_shallowest(const BkParams &pr, Sub &s, ui
       }
       w += popc64(m);
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    if (L == 0) __hip_atomic_store(&pr.seq[pos % pr.ring_cap], pos + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    publish_task(pr, rcv);
     // the donor treats c as explored: P -> X in frame j
     const uint64_t m = 1ull << bit;
     if (j == d) {
@@ -834,11 +918,11 @@ __device__ __forceinline__ void bk_solve(const BkParams &pr, Sub &s, uint64_t cP
 // E (WX words) and {pivot, mix}. Row index order is vertex-id order, so the
 // lowest-index tie-break equals the list path's id tie-break.
 
-__device__ __forceinline__ uint64_t bits_fixed(uint32_t p, uint32_t q, uint32_t W) {
+__host__ __device__ __forceinline__ uint64_t bits_fixed(uint32_t p, uint32_t q, uint32_t W) {
   return align8(4ull * p) + 8ull * p * W + 8ull * q * W;
 }
 
-__device__ __forceinline__ uint32_t bits_fs(uint32_t W, uint32_t WX) { return 8 * (2 * W + WX) + 16; }
+__host__ __device__ __forceinline__ uint32_t bits_fs(uint32_t W, uint32_t WX) { return 8 * (2 * W + WX) + 16; }
 
 // Lays out a bitmap subproblem over an LDS region of lds_bytes (taken from
 // the workgroup pool) and the wave's HBM arena. In order of preference: all
@@ -893,16 +977,16 @@ __device__ __forceinline__ bool place_bits(Sub &s, uint32_t p, uint32_t q, uint3
 
 // LDS bytes a bitmap subproblem asks the pool for: the whole layout plus
 // `levels` frames.
-__device__ __forceinline__ uint32_t bits_want(uint32_t p, uint32_t q, uint32_t W, uint32_t levels) {
+__host__ __device__ __forceinline__ uint32_t bits_want(uint32_t p, uint32_t q, uint32_t W, uint32_t levels) {
   const uint32_t WX = (q + 63) / 64;
-  return (uint32_t)min<uint64_t>(bits_fixed(p, q, W) + (uint64_t)bits_fs(W, WX) * min(p, levels), 0xFFFFFFFFull);
+  return (uint32_t)min64(bits_fixed(p, q, W) + (uint64_t)bits_fs(W, WX) * min64(p, levels), 0xFFFFFFFFull);
 }
 
 // ... with the outside rows left in HBM.
-__device__ __forceinline__ uint32_t bits_want_inner(uint32_t p, uint32_t q, uint32_t W, uint32_t levels) {
+__host__ __device__ __forceinline__ uint32_t bits_want_inner(uint32_t p, uint32_t q, uint32_t W, uint32_t levels) {
   const uint32_t WX = (q + 63) / 64;
-  return (uint32_t)min<uint64_t>(align8(4ull * p) + 8ull * p * W + (uint64_t)bits_fs(W, WX) * min(p, levels),
-                                 0xFFFFFFFFull);
+  return (uint32_t)min64(align8(4ull * p) + 8ull * p * W + (uint64_t)bits_fs(W, WX) * min64(p, levels),
+                         0xFFFFFFFFull);
 }
 
 // HBM arena bytes a bitmap subproblem needs when the pool gives it no LDS.
@@ -1072,7 +1156,7 @@ template <bool kLA, bool kLX>
 __device__ __forceinline__ bool donate_bits(const BkParams &pr, Sub &s, uint32_t d, uint64_t &cP, uint64_t &cX, uint64_t cE,
                             uint32_t cpiv, uint64_t cmix, WaveAcc &wa) {
   const uint32_t L = lane_id(), W = s.W, WX = s.WX;
-  const bool act = L < W, actx = L < WX;
+  const bool act = L < W;
   // a receiver may get no LDS from its pool: it must fit the HBM arena alone
   if (s.p > pr.slot_pmax || s.q > pr.slot_xcap || bits_need(s.p, s.q) > pr.hbm_bytes) return false;
   for (uint32_t j = 0; j <= d; ++j) {
@@ -1097,21 +1181,16 @@ __device__ __forceinline__ bool donate_bits(const BkParams &pr, Sub &s, uint32_t
     const uint64_t Pn = Pj & Ac, Xn = Xj & Ac;
     if (wave_sum(popc64(Pn)) < pr.donate_min_p) continue;
     const uint64_t En = filter_rows<kLX>(s, Ej, c);
-    unsigned long long pos = 0;
-    if (L == 0) {
-      pos = ring_claim_push(pr);
-      if (pos != kNoSlot) atomicAdd(pr.ctl + kCtlOutstanding, 1ull);
-    }
-    pos = uniform64(pos);
-    if (pos == kNoSlot) return false;
-    uint8_t *slot = pr.ring + (pos % pr.ring_cap) * (uint64_t)pr.slot_bytes;
+    const uint32_t rcv = claim_receiver(pr);
+    if (rcv == kNone) return false;
+    uint8_t *slot = pr.ring + (uint64_t)rcv * pr.slot_bytes;
     if (L == 0) {
       uint32_t *h = (uint32_t *)slot;
       h[0] = s.root;
       h[1] = s.rsize + j + 1;
       h[2] = s.p;
       h[3] = W;
-      h[4] = s.q;
+      h[4] = 0;  // outside rows: set once they are packed
       h[5] = kTaskBits;
       *(uint64_t *)(slot + 24) = mixj + splitmix64(s.S[c]);
     }
@@ -1120,17 +1199,24 @@ __device__ __forceinline__ bool donate_bits(const BkParams &pr, Sub &s, uint32_t
       sp[L] = Pn;
       sp[pr.slot_wmax + L] = Xn;
     }
-    if (actx) sp[2 * pr.slot_wmax + L] = En;
-    uint32_t *sS = (uint32_t *)(slot + pr.slot_oS);
-    for (uint32_t i = L; i < s.p; i += kWave) sS[i] = s.S[i];
-    uint64_t *sA = (uint64_t *)(slot + pr.slot_oA);
-    const uint64_t nA = (uint64_t)s.p * W;
-    for (uint64_t i = L; i < nA; i += kWave) sA[i] = s.A[i];
+    wave_copy((uint32_t *)(slot + pr.slot_oS), s.S, s.p);
+    copy64((uint64_t *)(slot + pr.slot_oA), s.A, (uint64_t)s.p * W);
+    // only the outside rows live in the child travel, in row order (which
+    // keeps the pivot's tie rule), renumbered 0..q'-1
     uint64_t *sXR = (uint64_t *)(slot + pr.slot_oXR);
-    const uint64_t nR = (uint64_t)s.q * W;
-    for (uint64_t i = L; i < nR; i += kWave) sXR[i] = s.XR[i];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    if (L == 0) __hip_atomic_store(&pr.seq[pos % pr.ring_cap], pos + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t qn = 0;
+    for (uint64_t em = ballot(En != 0); em; em &= em - 1) {
+      const uint32_t k = ctz64(em);
+      const uint64_t ek = rdlane64(En, k);
+      if ((ek >> L) & 1ull) {
+        const uint64_t src = (uint64_t)(k * 64 + L) * W, dst = (uint64_t)(qn + mbcnt(ek)) * W;
+        for (uint32_t w = 0; w < W; ++w) sXR[dst + w] = ld64<kLX>(s.XR, src + w);
+      }
+      qn += popc64(ek);
+    }
+    if (L < (qn + 63) / 64) sp[2 * pr.slot_wmax + L] = full_word(qn);
+    if (L == 0) ((uint32_t *)slot)[4] = qn;
+    publish_task(pr, rcv);
     const uint64_t m = 1ull << bit;
     if (j == d) {
       if (L == k0) {
@@ -1263,15 +1349,6 @@ __device__ __forceinline__ uint32_t intersect_into(const uint32_t *a, uint32_t n
   return w;
 }
 
-// Forward copy of n words; each chunk of 64 is read before it is written, so a
-// destination below an overlapping source is safe.
-__device__ __forceinline__ void wave_copy(uint32_t *dst, const uint32_t *src, uint32_t n) {
-  for (uint32_t b = 0; b < n; b += kWave) {
-    const uint32_t i = b + lane_id();
-    const uint32_t x = i < n ? src[i] : 0;
-    if (i < n) dst[i] = x;
-  }
-}
 
 __device__ __forceinline__ uint32_t hw_cu_id() {
   // HW_ID (hwreg 4): CU_ID [11:8], SH_ID [12], SE_ID [15:13]; XCC_ID (hwreg 20) [3:0]
@@ -1290,12 +1367,6 @@ __device__ __forceinline__ void defer(const BkParams &pr, uint64_t it) {
   }
 }
 
-// All-ones P over the first p bits (lane k holds word k).
-__device__ __forceinline__ uint64_t full_word(uint32_t p) {
-  const uint32_t L = lane_id(), W = (p + 63) / 64;
-  if (L >= W) return 0;
-  return (L == W - 1 && (p & 63)) ? ((1ull << (p & 63)) - 1) : ~0ull;
-}
 
 template <bool kIPX, bool kWL, int kStat>
 __device__ __forceinline__ void finish_sub(const BkParams &pr, Sub &s, uint64_t P0, uint64_t X0, unsigned long long *hist_lds,
@@ -1326,7 +1397,9 @@ __device__ __forceinline__ void finish_bits(const BkParams &pr, Sub &s, const ui
 }
 
 template <bool kIPX, bool kWL, int kStat>
-__global__ __launch_bounds__(kMaxWpg * kWave) void bk_kernel(BkParams pr) {
+// At least four waves per SIMD (<= 128 VGPRs): occupancy hides the LDS and
+// memory latency of the set operations better than registers would.
+__global__ __launch_bounds__(kMaxWpg * kWave, 4) void bk_kernel(BkParams pr) {
   extern __shared__ __align__(16) uint8_t smem[];
   unsigned long long *hist_lds = (unsigned long long *)smem;
   const uint32_t wig = threadIdx.x / kWave, L = lane_id();
@@ -1382,9 +1455,10 @@ __global__ __launch_bounds__(kMaxWpg * kWave) void bk_kernel(BkParams pr) {
         if (p == 0) continue;
         const uint32_t W = (p + 63) / 64;
         const bool bits = kIPX && q <= kMaxXRows;
-        const Region rg = bits ? pool_take(pl, bits_want(p, q, W, lv), bits_want_inner(p, q, W, lv),
-                                           bits_fs(W, (q + 63) / 64) * min(p, lv))
-                               : pool_take(pl, list_want(p, q, W, kIPX, lv), 0, (16 * W + 16) * min(p, lv));
+        const Region rg = bits ? pool_take_wait(pl, bits_want(p, q, W, lv), bits_want_inner(p, q, W, lv),
+                                                bits_fs(W, (q + 63) / 64) * min(p, lv), bits_want_inner(p, q, W, lv))
+                               : pool_take_wait(pl, list_want(p, q, W, kIPX, lv), 0, (16 * W + 16) * min(p, lv),
+                                                list_want(p, q, W, kIPX, lv));
         if (!(bits ? place_bits(s, p, q, W, rg.ptr, rg.bytes, hbm, pr.hbm_bytes, pr.stack_lds)
                    : place(s, p, q, W, kIPX, rg.ptr, rg.bytes, hbm, pr.hbm_bytes, pr.stack_lds))) {
           pool_put(pl, rg);
@@ -1413,7 +1487,8 @@ __global__ __launch_bounds__(kMaxWpg * kWave) void bk_kernel(BkParams pr) {
         const uint32_t pb = (uint32_t)min(eu - su, ew - sw), qb = (uint32_t)min(eu - bu, sw - bw);
         Sub t = s;
         const uint32_t Wb = (pb + 63) / 64;
-        const Region rg = pool_take(pl, list_want(pb, qb, Wb, kIPX, lv), 0, (16 * Wb + 16) * min(pb, lv));
+        const Region rg = pool_take_wait(pl, list_want(pb, qb, Wb, kIPX, lv), 0, (16 * Wb + 16) * min(pb, lv),
+                                         list_want(pb, qb, Wb, kIPX, lv));
         if (!place(t, pb, qb, Wb, kIPX, rg.ptr, rg.bytes, hbm, pr.hbm_bytes, pr.stack_lds)) {
           pool_put(pl, rg);
           defer(pr, it);
@@ -1454,146 +1529,152 @@ __global__ __launch_bounds__(kMaxWpg * kWave) void bk_kernel(BkParams pr) {
     }
     if (kWL && L == 0) atomicSub(pr.ctl + kCtlOutstanding, 1ull);
   }
-  // phase 2: donated subtrees until no work is outstanding
+  // phase 2: donated subtrees until no work is outstanding. An idle wave puts
+  // itself on the idle queue and waits on its own mailbox line; a donor takes
+  // it off the queue and writes the task into its slot, so no line is polled
+  // by many waves. The receiver takes LDS from its pool once the task is in
+  // (holding LDS while idle would starve the waves still at work).
   if constexpr (kWL) {
-    bool idle = false;
-    uint32_t backoff = 1;
+    bool registered = false;
+    Region rsv{pl.base, 0, 0, 0};
+    uint32_t backoff = 1, polls = 0;
     uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    const uint8_t *slot = pr.ring + (uint64_t)gw * pr.slot_bytes;
     while (true) {
       clk.to(kBdQueue);
-      unsigned long long pos = 0;
-      if (L == 0) pos = ring_claim_pop(pr);
-      pos = uniform64(pos);
-      if (pos != kNoSlot) {
-        if (idle && L == 0) atomicSub(pr.ctl + kCtlIdle, 1ull);
-        idle = false;
-        backoff = 1;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        const uint8_t *slot = pr.ring + (pos % pr.ring_cap) * (uint64_t)pr.slot_bytes;
-        const uint32_t *h = (const uint32_t *)slot;
-        const uint32_t root = uniform(h[0]), rsize = uniform(h[1]), p = uniform(h[2]), W = uniform(h[3]),
-                       nx = uniform(h[4]), kind = uniform(h[5]);
-        const uint64_t rmix = uniform64(*(const uint64_t *)(slot + 24));
-        if (kIPX && kind == kTaskBits) {
-          const bool shape = p <= pr.slot_pmax && W == (p + 63) / 64 && nx <= pr.slot_xcap && nx <= kMaxXRows;
-          const Region rg = shape ? pool_take(pl, bits_want(p, nx, W, lv), bits_want_inner(p, nx, W, lv),
-                                              bits_fs(W, (nx + 63) / 64) * min(p, lv))
-                                  : Region{pl.base, 0, 0, 0};
-          const bool okb = shape && place_bits(s, p, nx, W, rg.ptr, rg.bytes, hbm, pr.hbm_bytes, pr.stack_lds);
-          uint64_t P0 = 0, X0 = 0, E0 = 0;
-          if (okb) {
+      if (!registered) registered = register_idle(pr, gw);
+      if (registered) {
+        const unsigned long long got = uniform64(
+            L == 0 ? __hip_atomic_load(&pr.mbox[(uint64_t)gw * kMboxStride], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)
+                   : 0ull);
+        if (got) {
+          registered = false;
+          backoff = 1;
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+          const uint32_t *h = (const uint32_t *)slot;
+          const uint32_t root = uniform(h[0]), rsize = uniform(h[1]), p = uniform(h[2]), W = uniform(h[3]),
+                         nx = uniform(h[4]), kind = uniform(h[5]);
+          const uint64_t rmix = uniform64(*(const uint64_t *)(slot + 24));
+          if (kIPX && kind == kTaskBits) {
+            const bool shape = p <= pr.slot_pmax && W == (p + 63) / 64 && nx <= pr.slot_xcap && nx <= kMaxXRows;
+            if (shape)
+              rsv = pool_take_wait(pl, bits_want(p, nx, W, lv), bits_want_inner(p, nx, W, lv),
+                                   bits_fs(W, (nx + 63) / 64) * min(p, lv), bits_want_inner(p, nx, W, lv));
+            const Region rg = rsv;
+            const bool okb = shape && place_bits(s, p, nx, W, rg.ptr, rg.bytes, hbm, pr.hbm_bytes, pr.stack_lds);
+            uint64_t P0 = 0, X0 = 0, E0 = 0;
+            if (okb) {
+              const uint64_t *sp = (const uint64_t *)(slot + pr.slot_oP);
+              if (L < W) {
+                P0 = sp[L];
+                X0 = sp[pr.slot_wmax + L];
+              }
+              if (L < s.WX) E0 = sp[2 * pr.slot_wmax + L];
+              wave_copy(s.S, (const uint32_t *)(slot + pr.slot_oS), p);
+              const uint64_t *sA = (const uint64_t *)(slot + pr.slot_oA);
+              const uint64_t nA = (uint64_t)p * W;
+              copy64(s.A, sA, nA);
+              const uint64_t *sXR = (const uint64_t *)(slot + pr.slot_oXR);
+              const uint64_t nR = (uint64_t)nx * W;
+              copy64(s.XR, sXR, nR);
+            }
+            if (okb) {
+              __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+              s.p = p;
+              s.q = nx;
+              s.W = W;
+              s.rsize = rsize;
+              s.root = root;
+              s.rmix = rmix;
+              wa.sub_cliques = 0;
+              wa.t_task = __builtin_amdgcn_s_memrealtime();
+              const uint64_t nodes0 = wa.nodes;
+              solve_bits<kWL, kStat>(pr, s, P0, X0, E0, hist_lds, wa, clk);
+              clk.to(kBdOther);
+              if (L == 0) {
+                if (pr.sub_nodes) atomicAdd(&pr.sub_nodes[root], (unsigned long long)(wa.nodes - nodes0));
+                if (pr.sub_cliques && wa.sub_cliques)
+                  atomicAdd(&pr.sub_cliques[root], (unsigned long long)wa.sub_cliques);
+              }
+            } else if (L == 0) {
+              atomicOr(&pr.acc[kAccError], 2ull);
+            }
+            if (L == 0) atomicSub(pr.ctl + kCtlOutstanding, 1ull);
+            pool_put(pl, rsv);
+            rsv = Region{pl.base, 0, 0, 0};
+            t0 = __builtin_amdgcn_s_memrealtime();
+            continue;
+          }
+          const bool shape = kind == kTaskList && p <= pr.slot_pmax && W == (p + 63) / 64 && nx <= pr.slot_xcap;
+          if (shape)
+            rsv = pool_take_wait(pl, list_want(p, nx, W, kIPX, lv), 0, (16 * W + 16) * min(p, lv),
+                                 list_want(p, nx, W, kIPX, lv));
+          const Region rg = rsv;
+          const bool ok = shape && place(s, p, nx, W, kIPX, rg.ptr, rg.bytes, hbm, pr.hbm_bytes, pr.stack_lds);
+          uint64_t P0 = 0, X0 = 0;
+          if (ok) {
             const uint64_t *sp = (const uint64_t *)(slot + pr.slot_oP);
             if (L < W) {
               P0 = sp[L];
               X0 = sp[pr.slot_wmax + L];
             }
-            if (L < s.WX) E0 = sp[2 * pr.slot_wmax + L];
             wave_copy(s.S, (const uint32_t *)(slot + pr.slot_oS), p);
             const uint64_t *sA = (const uint64_t *)(slot + pr.slot_oA);
             const uint64_t nA = (uint64_t)p * W;
-            for (uint64_t i = L; i < nA; i += kWave) s.A[i] = sA[i];
-            const uint64_t *sXR = (const uint64_t *)(slot + pr.slot_oXR);
-            const uint64_t nR = (uint64_t)nx * W;
-            for (uint64_t i = L; i < nR; i += kWave) s.XR[i] = sXR[i];
+            copy64(s.A, sA, nA);
+            const uint32_t *sXI = (const uint32_t *)(slot + pr.slot_oXI);
+            if (kIPX) {
+              wave_copy(s.XV, sXI, nx);
+              const uint64_t *sXR = (const uint64_t *)(slot + pr.slot_oXR);
+              const uint64_t nR = (uint64_t)nx * W;
+              copy64(s.XR, sXR, nR);
+              for (uint32_t i = L; i < nx; i += kWave) s.XL[i] = i;
+            } else {
+              wave_copy(s.XL, sXI, nx);
+            }
           }
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-          if (L == 0)
-            __hip_atomic_store(&pr.seq[pos % pr.ring_cap], pos + pr.ring_cap, __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
-          if (okb) {
+          if (ok) {
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
             s.p = p;
             s.q = nx;
             s.W = W;
+            s.xe0 = nx;
             s.rsize = rsize;
             s.root = root;
             s.rmix = rmix;
             wa.sub_cliques = 0;
             wa.t_task = __builtin_amdgcn_s_memrealtime();
             const uint64_t nodes0 = wa.nodes;
-            solve_bits<kWL, kStat>(pr, s, P0, X0, E0, hist_lds, wa, clk);
+            bk_solve<kIPX, kWL, kStat>(pr, s, P0, X0, hist_lds, wa, clk);
             clk.to(kBdOther);
             if (L == 0) {
               if (pr.sub_nodes) atomicAdd(&pr.sub_nodes[root], (unsigned long long)(wa.nodes - nodes0));
-              if (pr.sub_cliques && wa.sub_cliques)
-                atomicAdd(&pr.sub_cliques[root], (unsigned long long)wa.sub_cliques);
+              if (pr.sub_cliques && wa.sub_cliques) atomicAdd(&pr.sub_cliques[root], (unsigned long long)wa.sub_cliques);
             }
           } else if (L == 0) {
-            atomicOr(&pr.acc[kAccError], 2ull);
+            atomicOr(&pr.acc[kAccError], 2ull);  // donors only publish what fits
           }
-          pool_put(pl, rg);
           if (L == 0) atomicSub(pr.ctl + kCtlOutstanding, 1ull);
+          pool_put(pl, rsv);
+          rsv = Region{pl.base, 0, 0, 0};
           t0 = __builtin_amdgcn_s_memrealtime();
           continue;
         }
-        const bool shape = kind == kTaskList && p <= pr.slot_pmax && W == (p + 63) / 64 && nx <= pr.slot_xcap;
-        const Region rg = shape ? pool_take(pl, list_want(p, nx, W, kIPX, lv), 0, (16 * W + 16) * min(p, lv))
-                                : Region{pl.base, 0, 0, 0};
-        const bool ok = shape && place(s, p, nx, W, kIPX, rg.ptr, rg.bytes, hbm, pr.hbm_bytes, pr.stack_lds);
-        uint64_t P0 = 0, X0 = 0;
-        if (ok) {
-          const uint64_t *sp = (const uint64_t *)(slot + pr.slot_oP);
-          if (L < W) {
-            P0 = sp[L];
-            X0 = sp[pr.slot_wmax + L];
-          }
-          wave_copy(s.S, (const uint32_t *)(slot + pr.slot_oS), p);
-          const uint64_t *sA = (const uint64_t *)(slot + pr.slot_oA);
-          const uint64_t nA = (uint64_t)p * W;
-          for (uint64_t i = L; i < nA; i += kWave) s.A[i] = sA[i];
-          const uint32_t *sXI = (const uint32_t *)(slot + pr.slot_oXI);
-          if (kIPX) {
-            wave_copy(s.XV, sXI, nx);
-            const uint64_t *sXR = (const uint64_t *)(slot + pr.slot_oXR);
-            const uint64_t nR = (uint64_t)nx * W;
-            for (uint64_t i = L; i < nR; i += kWave) s.XR[i] = sXR[i];
-            for (uint32_t i = L; i < nx; i += kWave) s.XL[i] = i;
-          } else {
-            wave_copy(s.XL, sXI, nx);
-          }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        if (L == 0)
-          __hip_atomic_store(&pr.seq[pos % pr.ring_cap], pos + pr.ring_cap, __ATOMIC_RELEASE,
-                             __HIP_MEMORY_SCOPE_AGENT);
-        if (ok) {
-          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-          s.p = p;
-          s.q = nx;
-          s.W = W;
-          s.xe0 = nx;
-          s.rsize = rsize;
-          s.root = root;
-          s.rmix = rmix;
-          wa.sub_cliques = 0;
-          wa.t_task = __builtin_amdgcn_s_memrealtime();
-          const uint64_t nodes0 = wa.nodes;
-          bk_solve<kIPX, kWL, kStat>(pr, s, P0, X0, hist_lds, wa, clk);
-          clk.to(kBdOther);
-          if (L == 0) {
-            if (pr.sub_nodes) atomicAdd(&pr.sub_nodes[root], (unsigned long long)(wa.nodes - nodes0));
-            if (pr.sub_cliques && wa.sub_cliques) atomicAdd(&pr.sub_cliques[root], (unsigned long long)wa.sub_cliques);
-          }
-        } else if (L == 0) {
-          atomicOr(&pr.acc[kAccError], 2ull);  // donors only publish what fits
-        }
-        pool_put(pl, rg);
-        if (L == 0) atomicSub(pr.ctl + kCtlOutstanding, 1ull);
-        t0 = __builtin_amdgcn_s_memrealtime();
-        continue;
       }
-      if (!idle && L == 0) atomicAdd(pr.ctl + kCtlIdle, 1ull);
-      idle = true;
-      const unsigned long long out = uniform64(L == 0 ? ld_relaxed(pr.ctl + kCtlOutstanding) : 0);
-      if (out == 0) break;
+      // the shared outstanding count is read every 16th poll only
+      if ((++polls & 15) == 0 || !registered) {
+        const unsigned long long out = uniform64(L == 0 ? ld_relaxed(pr.ctl + kCtlOutstanding) : 0);
+        if (out == 0) break;
+      }
       if (__builtin_amdgcn_s_memrealtime() - t0 > kWatchdogTicks) {
         if (L == 0) atomicOr(&pr.acc[kAccError], 1ull);
         break;
       }
-      // back off up to ~64 x 2K cycles between polls of the shared ring lines
+      // a registered wave polls only its own line: short sleeps
       for (uint32_t z = 0; z < backoff; ++z) __builtin_amdgcn_s_sleep(32);
-      backoff = min(backoff * 2, 64u);
+      backoff = min(backoff * 2, registered ? 16u : 64u);
     }
+    pool_put(pl, rsv);
   }
   clk.to(kBdOther);
   if (L == 0) {
@@ -1621,9 +1702,12 @@ __global__ void reset_kernel(unsigned long long *acc, uint32_t n, uint32_t *tick
   if (threadIdx.x == 0) *ticket = 0;
 }
 
-__global__ void ring_init(unsigned long long *seq, uint32_t cap, unsigned long long *ctl,
-                          unsigned long long outstanding) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) seq[i] = i;
+__global__ void ring_init(unsigned long long *seq, uint32_t cap, unsigned long long *ctl, unsigned long long outstanding,
+                          unsigned long long *mbox) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) {
+    seq[i] = i;
+    mbox[(uint64_t)i * kMboxStride] = 0;
+  }
   if (blockIdx.x == 0 && threadIdx.x < kCtlWords) ctl[threadIdx.x] = threadIdx.x == kCtlOutstanding ? outstanding : 0;
 }
 
@@ -1859,7 +1943,9 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
     if ((rc = dmalloc((void **)&pr.ctl, sizeof(unsigned long long) * kCtlWords))) return rc;
     if ((rc = dmalloc((void **)&pr.seq, sizeof(unsigned long long) * pr.ring_cap))) return rc;
     if ((rc = dmalloc((void **)&pr.ring, pr.slot_bytes * pr.ring_cap))) return rc;
-    if (!cfg.capture) ring_init<<<64, 256, 0, s>>>(pr.seq, pr.ring_cap, pr.ctl, n_tickets);
+    if ((rc = dmalloc((void **)&pr.idle_ids, sizeof(uint32_t) * pr.ring_cap))) return rc;
+    if ((rc = dmalloc((void **)&pr.mbox, sizeof(unsigned long long) * kMboxStride * pr.ring_cap))) return rc;
+    if (!cfg.capture) ring_init<<<64, 256, 0, s>>>(pr.seq, pr.ring_cap, pr.ctl, n_tickets, pr.mbox);
   }
 
   hipEvent_t e0, e1;
@@ -1873,7 +1959,7 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
     hipGraphExec_t exec = nullptr;
     MCE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     reset_kernel<<<1, 64, 0, s>>>(acc, kAccN, ticket);
-    if (cfg.wl) ring_init<<<64, 256, 0, s>>>(pr.seq, pr.ring_cap, pr.ctl, n_tickets);
+    if (cfg.wl) ring_init<<<64, 256, 0, s>>>(pr.seq, pr.ring_cap, pr.ctl, n_tickets, pr.mbox);
     rc = dispatch(cfg.ipx, cfg.wl, stat, false, nullptr, pr, groups, wpg, lds, s);
     const hipError_t ec = hipStreamEndCapture(s, &graph);
     if (rc) return rc;
