@@ -33,6 +33,15 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 METRIC = "maximal cliques/s on 1 MI355X: learned-scheduled vs plain BK, count bit-exact"
 HBM_PEAK_GBS = 8000.0
+SCHEDULES = {"plain": 0, "learned": 1}
+
+
+def spearman(a, b):
+    ra = np.argsort(np.argsort(a, kind="stable"), kind="stable").astype(np.float64)
+    rb = np.argsort(np.argsort(b, kind="stable"), kind="stable").astype(np.float64)
+    ra -= ra.mean()
+    rb -= rb.mean()
+    return float((ra * rb).sum() / np.sqrt((ra * ra).sum() * (rb * rb).sum()))
 
 
 def _import(name, path):
@@ -116,7 +125,10 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--level", type=int, default=1)
     ap.add_argument("--induced", type=int, default=2)
-    ap.add_argument("--worklist", type=int, default=2)
+    ap.add_argument("--worklist", type=int, default=3, help="1 nowl, 2 wl, 3 auto (picked by the cost model)")
+    ap.add_argument("--schedule", choices=["learned", "plain"], default="learned")
+    ap.add_argument("--model", default=os.path.join(ROOT, "mce-gpu_amd", "cost_model_sbm.mlp"))
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-schedule comparison step")
     ap.add_argument("--cpu-sample", type=float, default=1.0 / 64,
                     help="fraction of member graphs (batch) or first-level vertices timed on the CPU")
     ap.add_argument("--no-cpu", action="store_true")
@@ -133,7 +145,9 @@ def main():
     g, off, desc = make_workload(mce, args.config, args.seed)
     log(f"[bench] generated {args.config}: n={g.n} arcs={g.m} in {time.time() - t0:.1f}s")
 
-    kw = dict(level=args.level, induced=args.induced, worklist=args.worklist)
+    model_path = args.model if args.schedule == "learned" else None
+    kw = dict(level=args.level, induced=args.induced, worklist=args.worklist,
+              schedule=SCHEDULES[args.schedule], model_path=model_path)
     with mce.DeviceGraph(g) as dg:
         for w in range(args.warmup):
             info = dg.order()
@@ -150,12 +164,34 @@ def main():
             kern_ms.append(r["t_count_ms"])
             order_ms.append(info["t_order_ms"] + info["t_orient_ms"])
             log(f"[bench] step {k}: {step_ms[-1]:.1f} ms (order+orient {order_ms[-1]:.1f}, enum kernel "
-                f"{kern_ms[-1]:.1f}) count={r['count']} donations={r['donations']}")
+                f"{kern_ms[-1]:.1f}) count={r['count']} donations={r['donations']} waves={r['num_waves']} "
+                f"fallbacks={r['fallback_events']}")
             if first is None:
                 first = r
             elif (r["count"], r["hash"]) != (first["count"], first["hash"]):
                 raise SystemExit("steps disagree: results are not reproducible")
         _, split, _ = dg.download(prio=False, split=True, col=False)
+        plain = None
+        if not args.no_plain:
+            # the plain-schedule engine (heaviest-first by out-degree, worker
+            # list, default settings) on the same resident graph, with
+            # per-subproblem tree sizes to score the cost model's ranking
+            t = time.perf_counter()
+            dg.order()
+            rp_ = dg.count(level=args.level, induced=args.induced, worklist=2, schedule=0, sub_nodes=True)
+            plain_ms = (time.perf_counter() - t) * 1e3
+            if (rp_["count"], rp_["hash"]) != (first["count"], first["hash"]):
+                raise SystemExit("plain and learned schedules disagree")
+            plain = {"value": rp_["count"] / (plain_ms / 1e3), "ms_per_step": plain_ms,
+                     "enum_kernel_ms": rp_["t_count_ms"], "worklist": "wl", "schedule": "plain"}
+            F = dg.features()
+            roots = F[:, 1] > 0
+            y = np.log1p(rp_["sub_nodes"].astype(np.float64))[roots]
+            plain["spearman_out_degree"] = spearman(F[roots, 1], y)
+            if model_path and os.path.exists(model_path):
+                pred = mce.mlp_predict(mce.mlp_load(model_path), F[roots])
+                plain["spearman_model"] = spearman(pred, y)
+            log(f"[bench] plain engine: {plain_ms:.1f} ms ({plain})")
     count = first["count"]
     ms = float(np.mean(step_ms))
     value = count / (ms / 1e3)
@@ -167,16 +203,17 @@ def main():
         sc = first["sub_cliques"].astype(np.int64)
         per_graph = np.add.reduceat(sc, off[:-1].astype(np.int64)) if g.n else np.zeros(0)
         assert int(per_graph.sum()) == count
-
     out = {
         "metric": METRIC, "value": value, "unit": "maximal cliques/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u64", "data": "synthetic",
         "config": dict(desc, level=f"L{args.level}", induced="IPX" if args.induced == 2 else "IP",
-                       worklist="wl" if args.worklist == 2 else "nowl", schedule="plain",
+                       worklist={1: "nowl", 2: "wl", 3: "auto"}[args.worklist], schedule=args.schedule,
                        parallelism="1 GPU"),
         "count": count, "hash": f"{first['hash']:#018x}", "hist": first["hist"], "degeneracy": first["degeneracy"],
         "tree_nodes": first["tree_nodes"], "enum_kernel_ms": kavg, "order_orient_ms": float(np.mean(order_ms)),
+        "fallback_events": first["fallback_events"], "waves": first["num_waves"], "donations": first["donations"],
+        "plain_engine": plain, "speedup_vs_plain": (plain["ms_per_step"] / ms) if plain else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes": algo_b, "kernel": "bk_kernel (enumeration)"},

@@ -45,7 +45,13 @@ extern "C" {
 /* ---- enums mirroring the reference command line (include/defs.h:12-40) */
 enum mce_level { MCE_L1 = 1, MCE_L2 = 2 };       /* -p l1 | l2  */
 enum mce_induced { MCE_IP = 1, MCE_IPX = 2 };    /* -i p | px   */
-enum mce_worklist { MCE_NOWL = 1, MCE_WL = 2 };  /* -w nowl | wl */
+enum mce_worklist {                              /* -w nowl | wl | auto */
+  MCE_NOWL = 1,
+  MCE_WL = 2,
+  MCE_WL_AUTO = 3  /* the learned schedule picks: worker list when it predicts
+                      a subtree larger than a wave's share, else none; the
+                      plain schedule takes the worker list                   */
+};
 enum mce_task {                                  /* -m ...       */
   MCE_TASK_MCE = 0,        /* mce                                  */
   MCE_TASK_LB_EVAL = 1,    /* mce-lb-eval: tree nodes per wave       */

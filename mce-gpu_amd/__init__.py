@@ -21,7 +21,7 @@ CLI_PATH = os.path.join(_HERE, "bin", "parallel_mce_on_gpus")
 
 L1, L2 = 1, 2
 IP, IPX = 1, 2
-NOWL, WL = 1, 2
+NOWL, WL, WL_AUTO = 1, 2, 3
 TASK_MCE, TASK_LB_EVAL, TASK_BD_EVAL, TASK_DONOR_EVAL = 0, 1, 2, 3
 SCHED_PLAIN, SCHED_LEARNED, SCHED_GIVEN = 0, 1, 2
 MAX_OUT_DEGREE = 4096

@@ -70,8 +70,8 @@ void fill_order_fields(const mce_graph *h, mce_result *res) {
 int check_opts(const mce_opts *o) {
   if (o->level != MCE_L1 && o->level != MCE_L2) return set_error(MCE_EINVAL, "level must be MCE_L1 or MCE_L2");
   if (o->induced != MCE_IP && o->induced != MCE_IPX) return set_error(MCE_EINVAL, "induced must be MCE_IP or MCE_IPX");
-  if (o->worklist != MCE_NOWL && o->worklist != MCE_WL)
-    return set_error(MCE_EINVAL, "worklist must be MCE_NOWL or MCE_WL");
+  if (o->worklist != MCE_NOWL && o->worklist != MCE_WL && o->worklist != MCE_WL_AUTO)
+    return set_error(MCE_EINVAL, "worklist must be MCE_NOWL, MCE_WL or MCE_WL_AUTO");
   if (o->task < MCE_TASK_MCE || o->task > MCE_TASK_DONOR_EVAL) return set_error(MCE_EINVAL, "unknown task");
   // include/config.h:255-264
   if (o->task == MCE_TASK_BD_EVAL && o->worklist == MCE_NOWL)
@@ -269,7 +269,7 @@ static int count_impl(mce_graph *h, const mce_opts *opts, mce_result *res, bool 
     hipFree(arcs);
   };
   uint32_t len = 0;
-  bool heavy_tail = false;
+  bool heavy_tail = false, learned_used = false;
   if (g.n) {
     if (hipMalloc(&order, sizeof(uint32_t) * g.n) != hipSuccess) {
       cleanup();
@@ -310,6 +310,7 @@ static int count_impl(mce_graph *h, const mce_opts *opts, mce_result *res, bool 
           if (rc == MCE_OK && bad == 0) {
             rc = schedule_by_key(g, keys, 4095, order, &len, s);
             learned = rc == MCE_OK;
+            learned_used = learned;
             // a predicted subtree larger than a wave's share: donate early and small
             if (learned && wmax * 4096.0 > wsum) heavy_tail = true;
           }
@@ -354,7 +355,10 @@ static int count_impl(mce_graph *h, const mce_opts *opts, mce_result *res, bool 
   BkLaunch cfg;
   cfg.level = opts->level;
   cfg.ipx = opts->induced == MCE_IPX;
-  cfg.wl = opts->worklist == MCE_WL;
+  // the variant: a given one, or (auto) the worker list only where the cost
+  // model predicts a heavy tail
+  cfg.wl = opts->worklist == MCE_WL || (opts->worklist == MCE_WL_AUTO && (heavy_tail || !learned_used)) ||
+           opts->task == MCE_TASK_BD_EVAL || opts->task == MCE_TASK_DONOR_EVAL;
   cfg.task = opts->task;
   cfg.waves_per_group = (uint32_t)opts->waves_per_group;
   cfg.groups_per_cu = (uint32_t)opts->groups_per_cu;
@@ -364,11 +368,11 @@ static int count_impl(mce_graph *h, const mce_opts *opts, mce_result *res, bool 
   cfg.donate_check_nodes = opts->donate_check_nodes ? (uint32_t)opts->donate_check_nodes : 64;
   cfg.donate_age_us = opts->donate_age_us ? (uint32_t)opts->donate_age_us : 1000;
   cfg.capture = capture;
-  if (heavy_tail && !opts->donate_check_nodes) {
-    // thresholds set from the cost model: a heavy tail wants early, fine donation
-    cfg.donate_check_nodes = 16;
-    cfg.donate_min_p = std::min<uint32_t>(cfg.donate_min_p, 6);
-    if (!opts->donate_age_us) cfg.donate_age_us = 100;
+  if (heavy_tail && !opts->donate_age_us) {
+    // thresholds set from the cost model: with a predicted heavy tail only
+    // long-running tasks donate (a donation copies the donor's bit matrix;
+    // frequent small donations cost more than they balance)
+    cfg.donate_age_us = 10000;
   }
   BkOut out;
   const bool want_waves = opts->task == MCE_TASK_LB_EVAL || opts->task == MCE_TASK_DONOR_EVAL ||

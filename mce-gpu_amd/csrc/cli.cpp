@@ -136,6 +136,7 @@ int main(int argc, char **argv) {
       case 'w':
         if (!strcmp(optarg, "nowl")) worklist = MCE_NOWL;
         else if (!strcmp(optarg, "wl")) worklist = MCE_WL;
+        else if (!strcmp(optarg, "auto")) worklist = MCE_WL_AUTO;
         else {
           fprintf(stderr, "Unrecognized -w option (Worker List Scheme): %s\n", optarg);
           exit(0);
@@ -190,7 +191,8 @@ int main(int argc, char **argv) {
   } else {
     printf("    Parallelization Scheme = %s trees\n", level == MCE_L1 ? "L1" : "L2");
     printf("    Induced Subgraphs Scheme = %s\n", induced == MCE_IP ? "IP" : "IPX");
-    printf("    Worker List Scheme = %s\n", worklist == MCE_NOWL ? "No Worker List" : "Worker List");
+    printf("    Worker List Scheme = %s\n",
+           worklist == MCE_NOWL ? "No Worker List" : worklist == MCE_WL ? "Worker List" : "Auto (cost model)");
     printf("    Device ID(s) = %d\n", gpus[0]);
   }
   printf("-----------------------------\n");
