@@ -131,6 +131,7 @@ struct Sub {
   uint32_t *S;
   uint64_t *A;
   uint64_t *XR;
+  uint64_t *H;  // bitmap path: splitmix64(S[i]), the clique-hash term of S[i]
   uint32_t *XL;
   uint32_t *XA;
   uint32_t *XV;  // IPX: vertex id of each outside-X row
@@ -890,7 +891,9 @@ __device__ __forceinline__ void bk_solve(const BkParams &pr, Sub &s, uint64_t cP
     uint32_t npiv, nbest;
     clk.to(kBdPivot);
     pick_pivot<kIPX>(s, Pn, Xn, xn, npiv, nbest);
-    if (npiv == kNone || nbest == wave_sum(popc64(Pn))) continue;  // an X vertex covers P
+    // (a pivot from X covering P needs no test here: the child level finds
+    // P & ~N(pivot) empty at once and backtracks, counting no node)
+    if (npiv == kNone) continue;
     clk.to(kBdBranch);
     uint8_t *f = frame_at(s, d);
     if (act) {
@@ -919,7 +922,7 @@ __device__ __forceinline__ void bk_solve(const BkParams &pr, Sub &s, uint64_t cP
 // lowest-index tie-break equals the list path's id tie-break.
 
 __host__ __device__ __forceinline__ uint64_t bits_fixed(uint32_t p, uint32_t q, uint32_t W) {
-  return align8(4ull * p) + 8ull * p * W + 8ull * q * W;
+  return align8(4ull * p) + 8ull * p * W + 8ull * q * W + 8ull * p;
 }
 
 __host__ __device__ __forceinline__ uint32_t bits_fs(uint32_t W, uint32_t WX) { return 8 * (2 * W + WX) + 16; }
@@ -931,32 +934,38 @@ __host__ __device__ __forceinline__ uint32_t bits_fs(uint32_t W, uint32_t WX) { 
 // region holds in HBM. Frames past the LDS levels continue in HBM.
 __device__ __forceinline__ bool place_bits(Sub &s, uint32_t p, uint32_t q, uint32_t W, uint8_t *lds, uint32_t lds_bytes,
                                            uint8_t *hbm, uint64_t hbm_bytes, uint32_t stack_lds = 0) {
+  // layout: S | A | (outside rows) | H | frames; H after the rows, so that
+  // a level-2 staging list right after the rows stays intact while the rows
+  // are built (H is written last)
   const uint32_t WX = (q + 63) / 64;
-  const uint64_t sa = align8(4ull * p) + 8ull * p * W, xr = 8ull * q * W;
+  const uint64_t sa = align8(4ull * p) + 8ull * p * W, xr = 8ull * q * W, hb = 8ull * p;
   const uint32_t FS = bits_fs(W, WX);
   uint64_t need, in_lds;
-  uint8_t *xbase;
+  uint8_t *xbase, *hbase;
   uint8_t *base;
-  if (sa + xr + FS <= lds_bytes) {
+  if (sa + xr + hb + FS <= lds_bytes) {
     base = lds;
     xbase = lds + sa;
-    in_lds = sa + xr;
+    hbase = lds + sa + xr;
+    in_lds = sa + xr + hb;
     s.fr_hbm = hbm;
     need = 0;
     s.where = 3;
-  } else if (sa + FS <= lds_bytes) {
+  } else if (sa + hb + FS <= lds_bytes) {
     base = lds;
+    hbase = lds + sa;
     xbase = hbm;
-    in_lds = sa;
+    in_lds = sa + hb;
     s.fr_hbm = hbm + xr;
     need = xr;
     s.where = 1;
   } else {
     base = hbm;
     xbase = hbm + sa;
+    hbase = hbm + sa + xr;
     in_lds = 0;
-    s.fr_hbm = hbm + sa + xr;
-    need = sa + xr;
+    s.fr_hbm = hbm + sa + xr + hb;
+    need = sa + xr + hb;
     s.where = 0;
   }
   s.fr_lds = lds + in_lds;
@@ -969,6 +978,7 @@ __device__ __forceinline__ bool place_bits(Sub &s, uint32_t p, uint32_t q, uint3
   s.S = (uint32_t *)base;
   s.A = (uint64_t *)(base + align8(4ull * p));
   s.XR = (uint64_t *)xbase;
+  s.H = (uint64_t *)hbase;
   s.XL = nullptr;
   s.XA = nullptr;
   s.XV = nullptr;
@@ -985,7 +995,7 @@ __host__ __device__ __forceinline__ uint32_t bits_want(uint32_t p, uint32_t q, u
 // ... with the outside rows left in HBM.
 __host__ __device__ __forceinline__ uint32_t bits_want_inner(uint32_t p, uint32_t q, uint32_t W, uint32_t levels) {
   const uint32_t WX = (q + 63) / 64;
-  return (uint32_t)min64(align8(4ull * p) + 8ull * p * W + (uint64_t)bits_fs(W, WX) * min64(p, levels),
+  return (uint32_t)min64(align8(4ull * p) + 8ull * p * W + 8ull * p + (uint64_t)bits_fs(W, WX) * min64(p, levels),
                          0xFFFFFFFFull);
 }
 
@@ -993,6 +1003,11 @@ __host__ __device__ __forceinline__ uint32_t bits_want_inner(uint32_t p, uint32_
 __device__ __forceinline__ uint64_t bits_need(uint32_t p, uint32_t q) {
   const uint32_t W = (p + 63) / 64, WX = (q + 63) / 64;
   return bits_fixed(p, q, W) + (uint64_t)p * bits_fs(W, WX);
+}
+
+// H[i] = splitmix64(S[i]) for the subproblem's p vertices.
+__device__ __forceinline__ void fill_hash_terms(const Sub &s) {
+  for (uint32_t i = lane_id(); i < s.p; i += kWave) s.H[i] = splitmix64(s.S[i]);
 }
 
 // Builds A over S and the outside-X rows of the q vertices in `ids`; returns
@@ -1036,6 +1051,7 @@ __device__ __forceinline__ uint64_t build_bits(const BkParams &pr, Sub &s, const
     const uint64_t m = ballot(any);
     if (L == k) E0 = m;
   }
+  fill_hash_terms(s);
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
   return E0;
 }
@@ -1192,7 +1208,7 @@ __device__ __forceinline__ bool donate_bits(const BkParams &pr, Sub &s, uint32_t
       h[3] = W;
       h[4] = 0;  // outside rows: set once they are packed
       h[5] = kTaskBits;
-      *(uint64_t *)(slot + 24) = mixj + splitmix64(s.S[c]);
+      *(uint64_t *)(slot + 24) = mixj + s.H[c];
     }
     uint64_t *sp = (uint64_t *)(slot + pr.slot_oP);
     if (act) {
@@ -1289,7 +1305,7 @@ __device__ __forceinline__ void solve_bits_t(const BkParams &pr, Sub &s, uint64_
     const uint64_t Ac = act ? ld64<kLA>(s.A, (uint64_t)c * W + L) : 0;
     const uint64_t Pn = cP & Ac, Xn = cX & Ac;
     wa.nodes++;
-    const uint64_t cmix = mix + splitmix64(uniform(ld32<kLA>(s.S, c)));
+    const uint64_t cmix = mix + uniform64(ld64<kLA>(s.H, c));
     if (ballot(Pn != 0) == 0) {
       clk.to(kBdMaximal);
       if (ballot(Xn != 0) == 0 && !rows_any<kLX>(s, cE, c)) report(pr, hist_lds, wa, s.rsize + d + 1, cmix);
@@ -1299,7 +1315,9 @@ __device__ __forceinline__ void solve_bits_t(const BkParams &pr, Sub &s, uint64_
     uint32_t npiv, nbest;
     clk.to(kBdPivot);
     pick_pivot_bits<kLA, kLX>(s, Pn, Xn, En, npiv, nbest);
-    if (npiv == kNone || nbest == wave_sum(popc64(Pn))) continue;
+    // (a pivot from X covering P needs no test here: the child level finds
+    // P & ~N(pivot) empty at once and backtracks, counting no node)
+    if (npiv == kNone) continue;
     clk.to(kBdBranch);
     if (d < s.lds_levels) frame_put<true>(s.fr_lds + (uint64_t)d * s.FS, W, WX, cP, cX, cE, piv, mix);
     else frame_put<false>(s.fr_hbm + (uint64_t)(d - s.lds_levels) * s.FS, W, WX, cP, cX, cE, piv, mix);
@@ -1577,6 +1595,8 @@ __global__ __launch_bounds__(kMaxWpg * kWave, 4) void bk_kernel(BkParams pr) {
               const uint64_t *sXR = (const uint64_t *)(slot + pr.slot_oXR);
               const uint64_t nR = (uint64_t)nx * W;
               copy64(s.XR, sXR, nR);
+              s.p = p;
+              fill_hash_terms(s);
             }
             if (okb) {
               __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
