@@ -1306,12 +1306,76 @@ __device__ __forceinline__ void solve_bits_t(const BkParams &pr, Sub &s, uint64_
     const uint64_t Pn = cP & Ac, Xn = cX & Ac;
     wa.nodes++;
     const uint64_t cmix = mix + uniform64(ld64<kLA>(s.H, c));
-    if (ballot(Pn != 0) == 0) {
+    const uint64_t pnz = ballot(Pn != 0);
+    if (pnz == 0) {
       clk.to(kBdMaximal);
       if (ballot(Xn != 0) == 0 && !rows_any<kLX>(s, cE, c)) report(pr, hist_lds, wa, s.rsize + d + 1, cmix);
       continue;
     }
     const uint64_t En = filter_rows<kLX>(s, cE, c);
+    const uint64_t pnz2 = pnz & (pnz - 1);
+    if ((pnz2 & (pnz2 - 1)) == 0) {
+      // |P| <= 2 decided without the pivot scan, node for node as the scan
+      // would go (a pivot from X covering P leaves no child; ties go to the
+      // lowest index, inside X before outside rows)
+      const uint32_t k1 = ctz64(pnz);
+      const uint64_t w1 = rdlane64(Pn, k1);
+      const uint64_t w2 = pnz2 ? rdlane64(Pn, ctz64(pnz2)) : 0ull;
+      const uint32_t np = popc64(w1) + popc64(w2);
+      if (np == 1) {
+        // P = {a}: either an X vertex or outside row adjacent to a is the
+        // pivot and covers P, or a is the one child and a maximal clique
+        clk.to(kBdMaximal);
+        const uint32_t na = k1 * 64 + ctz64(w1);
+        const uint64_t Aa = act ? ld64<kLA>(s.A, (uint64_t)na * W + L) : 0;
+        if (ballot((Xn & Aa) != 0) == 0 && !rows_any<kLX>(s, En, na)) {
+          wa.nodes++;
+          report(pr, hist_lds, wa, s.rsize + d + 2, cmix + uniform64(ld64<kLA>(s.H, na)));
+        }
+        continue;
+      }
+      if (np == 2) {
+        clk.to(kBdMaximal);
+        const uint32_t na = k1 * 64 + ctz64(w1);
+        const uint32_t nb = w2 ? (uint32_t)ctz64(pnz2) * 64 + ctz64(w2) : k1 * 64 + ctz64(w1 & (w1 - 1));
+        const uint64_t Aa = act ? ld64<kLA>(s.A, (uint64_t)na * W + L) : 0;
+        const uint64_t Ab = act ? ld64<kLA>(s.A, (uint64_t)nb * W + L) : 0;
+        const uint64_t Ea = filter_rows<kLX>(s, En, na), Eb = filter_rows<kLX>(s, En, nb);
+        const uint64_t Xa = Xn & Aa, Xb = Xn & Ab;
+        if (ballot(((Xa & Xb) | (Ea & Eb)) != 0)) continue;  // a pivot covers P
+        const uint64_t ha = uniform64(ld64<kLA>(s.H, na)), hb = uniform64(ld64<kLA>(s.H, nb));
+        if ((rdlane64(Aa, nb >> 6) >> (nb & 63)) & 1ull) {
+          // a ~ b: whichever pivot, the path is a then b (or b then a)
+          wa.nodes += 2;
+          report(pr, hist_lds, wa, s.rsize + d + 3, cmix + ha + hb);
+          continue;
+        }
+        const uint64_t xs = Xa | Xb, es = Ea | Eb;
+        const uint64_t xm = ballot(xs != 0), em = ballot(es != 0);
+        if (xm == 0 && em == 0) {
+          // all scores 0: children a and b, each a maximal clique
+          wa.nodes += 2;
+          report(pr, hist_lds, wa, s.rsize + d + 2, cmix + ha);
+          report(pr, hist_lds, wa, s.rsize + d + 2, cmix + hb);
+          continue;
+        }
+        // the pivot is the lowest X vertex (else outside row) adjacent to
+        // exactly one of a, b; the child is the other, maximal iff nothing
+        // in X or outside touches it
+        bool piv_a;
+        if (xm) {
+          const uint32_t k = ctz64(xm);
+          piv_a = (rdlane64(Xa, k) >> ctz64(rdlane64(xs, k))) & 1ull;
+        } else {
+          const uint32_t k = ctz64(em);
+          piv_a = (rdlane64(Ea, k) >> ctz64(rdlane64(es, k))) & 1ull;
+        }
+        wa.nodes++;
+        const uint64_t Xc = piv_a ? Xb : Xa, Ec = piv_a ? Eb : Ea;
+        if (ballot((Xc | Ec) != 0) == 0) report(pr, hist_lds, wa, s.rsize + d + 2, cmix + (piv_a ? hb : ha));
+        continue;
+      }
+    }
     uint32_t npiv, nbest;
     clk.to(kBdPivot);
     pick_pivot_bits<kLA, kLX>(s, Pn, Xn, En, npiv, nbest);
