@@ -99,6 +99,22 @@ def sample_workload(mce, g, off, frac):
     return g, order, f"every {k}th first-level vertex ({order.size} of {g.n})"
 
 
+def measured_traffic(workload_key):
+    """HBM bytes of one enumeration launch from the committed counter summary
+    (profiles/bk_kernel_sbm4096_traffic.txt) when it covers this workload."""
+    path = os.path.join(ROOT, "profiles", "bk_kernel_sbm4096_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    kv = {}
+    for line in open(path):
+        if "=" in line and not line.startswith("#"):
+            k, v = line.split("=", 1)
+            kv[k.strip()] = v.strip()
+    if kv.get("workload") != workload_key:
+        return None
+    return float(kv["traffic_bytes"])
+
+
 def algorithmic_bytes(g, split, ipx=True):
     """HBM bytes the enumeration must stream at least once per launch: every
     first-level vertex reads its row and, to build its induced bit matrix, the
@@ -129,10 +145,25 @@ def main():
     ap.add_argument("--schedule", choices=["learned", "plain"], default="learned")
     ap.add_argument("--model", default=os.path.join(ROOT, "mce-gpu_amd", "cost_model_sbm.mlp"))
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-schedule comparison step")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default block settings")
+    ap.add_argument("--tune-sample", type=float, default=1.0 / 64,
+                    help="fraction of first-level subproblems the autotuner times each trial on")
     ap.add_argument("--cpu-sample", type=float, default=1.0 / 64,
                     help="fraction of member graphs (batch) or first-level vertices timed on the CPU")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run one device-to-device copy of BYTES (the reference kernel for the HBM "
+                         "traffic counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(args.copy_calibration // 8, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copied_bytes": a.numel() * 8}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the engine pins the device count to 1",
               file=sys.stderr)
@@ -154,6 +185,16 @@ def main():
             r = dg.count(**kw)
             log(f"[bench] warmup {w}: count={r['count']} order={info['t_order_ms']:.1f}ms "
                 f"count={r['t_count_ms']:.1f}ms")
+        tuned = None
+        if not args.no_autotune:
+            # block settings (and, left to it, the variant) timed on a sample
+            # of this graph; untimed set-up like the warmup
+            t = time.perf_counter()
+            best, trials = dg.autotune(sample_fraction=args.tune_sample, level=args.level, induced=args.induced,
+                                       worklist=args.worklist)
+            tuned = dict(best, trials=len(trials), seconds=time.perf_counter() - t)
+            kw.update(best)
+            log(f"[bench] autotuned in {tuned['seconds']:.1f}s over {len(trials)} trials: {best}")
         step_ms, kern_ms, order_ms = [], [], []
         first = None
         for k in range(args.steps):
@@ -198,6 +239,10 @@ def main():
     algo_b = algorithmic_bytes(g, split, ipx=args.induced == 2)
     kavg = float(np.mean(kern_ms))
     achieved = algo_b / (kavg / 1e3) / 1e9
+    traffic = None
+    if args.config == "c4":
+        traffic = measured_traffic(f"sbm-batch graphs=4096 n=64..4096 blocks=4 p_in=0.3 p_out=0.01 seed={args.seed} "
+                                   f"level={args.level} induced={args.induced}")
     per_graph = None
     if off is not None:
         sc = first["sub_cliques"].astype(np.int64)
@@ -214,8 +259,9 @@ def main():
         "tree_nodes": first["tree_nodes"], "enum_kernel_ms": kavg, "order_orient_ms": float(np.mean(order_ms)),
         "fallback_events": first["fallback_events"], "waves": first["num_waves"], "donations": first["donations"],
         "plain_engine": plain, "speedup_vs_plain": (plain["ms_per_step"] / ms) if plain else None,
+        "autotune": tuned,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes": algo_b, "kernel": "bk_kernel (enumeration)"},
     }
     if not args.no_cpu:

@@ -275,7 +275,8 @@ int mce_run_batch(const mce_batch *batch, const mce_opts *opts,
  * frame levels a subproblem asks the pool for; donation thresholds) on a
  * stride sample of the first-level subproblems of an ordered graph (at least
  * 64K of them, or all), heaviest first, and writes the fastest into *best;
- * report (nullable) receives a JSON list of every trial. Results
+ * report (nullable) receives a JSON list of every trial. With base worklist
+ * MCE_WL_AUTO the tuner also picks the variant (best->worklist). Results
  * do not depend on the settings; a trial that changes them is an error.
  * No reference counterpart (its block size is a fixed rule, mce.cuh:125-126). */
 int mce_autotune(mce_graph *g, const mce_opts *base, double sample_fraction,

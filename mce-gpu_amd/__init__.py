@@ -360,8 +360,8 @@ class DeviceGraph:
         best = Opts()
         buf = ctypes.create_string_buffer(1 << 16)
         _check(lib.mce_autotune(self.handle, byref(base), sample_fraction, byref(best), buf, len(buf)))
-        keys = ("waves_per_group", "groups_per_cu", "stack_lds_levels", "donate_min_p", "donate_check_nodes",
-                "donate_age_us")
+        keys = ("worklist", "waves_per_group", "groups_per_cu", "stack_lds_levels", "donate_min_p",
+                "donate_check_nodes", "donate_age_us")
         return {k: getattr(best, k) for k in keys}, _json.loads(buf.value.decode() or "[]")
 
     def close(self) -> None:

@@ -520,11 +520,14 @@ int mce_autotune(mce_graph *h, const mce_opts *base, double sample_fraction, mce
     return rp[x + 1] - split[x] > rp[y + 1] - split[y];
   });
   struct Cand {
-    int wpg, gpc, stack, minp, check, age;
+    int wl, wpg, gpc, stack, minp, check, age;
   };
-  // waves per workgroup x workgroups per CU; each workgroup shares its LDS
-  const Cand grid[] = {{0, 0, 0, 0, 0, 0}, {4, 1, 0, 0, 0, 0}, {8, 1, 0, 0, 0, 0}, {12, 1, 0, 0, 0, 0},
-                       {8, 2, 0, 0, 0, 0}, {6, 2, 0, 0, 0, 0}};
+  // the variant (both when the caller leaves it to the tuner), then waves per
+  // workgroup x workgroups per CU (each workgroup shares one LDS pool)
+  std::vector<int> variants;
+  if (base->worklist == MCE_WL_AUTO) variants = {MCE_NOWL, MCE_WL};
+  else variants = {base->worklist};
+  const int shapes[][2] = {{0, 0}, {4, 1}, {8, 1}, {12, 1}, {6, 2}};
   std::string rep = "[";
   mce_opts cur = *base;
   cur.schedule = MCE_SCHED_GIVEN;
@@ -534,12 +537,13 @@ int mce_autotune(mce_graph *h, const mce_opts *base, double sample_fraction, mce
   cur.lds_arena_bytes = 0;
   mce_result r{};
   double best_ms = 1e300;
-  Cand bestc{base->waves_per_group, base->groups_per_cu, base->stack_lds_levels, base->donate_min_p,
+  Cand bestc{variants[0], base->waves_per_group, base->groups_per_cu, base->stack_lds_levels, base->donate_min_p,
              base->donate_check_nodes, base->donate_age_us};
   uint64_t ref_count = 0, ref_hash = 0;
   bool have_ref = false;
   auto run = [&](const Cand &cd) -> int {
     mce_opts o = cur;
+    o.worklist = cd.wl;
     o.waves_per_group = cd.wpg;
     o.groups_per_cu = cd.gpc;
     o.stack_lds_levels = cd.stack;
@@ -557,9 +561,10 @@ int mce_autotune(mce_graph *h, const mce_opts *base, double sample_fraction, mce
       return set_error(MCE_EINTERNAL, "results changed with the block settings");
     }
     char buf[256];
-    snprintf(buf, sizeof(buf), "%s{\"waves_per_group\":%d,\"groups_per_cu\":%d,\"stack_lds_levels\":%d,"
-             "\"donate_min_p\":%d,\"donate_check_nodes\":%d,\"donate_age_us\":%d,\"waves\":%u,\"ms\":%.3f}",
-             rep.size() > 1 ? "," : "", cd.wpg, cd.gpc, cd.stack, cd.minp, cd.check, cd.age, r.num_waves,
+    snprintf(buf, sizeof(buf), "%s{\"worklist\":%d,\"waves_per_group\":%d,\"groups_per_cu\":%d,"
+             "\"stack_lds_levels\":%d,\"donate_min_p\":%d,\"donate_check_nodes\":%d,\"donate_age_us\":%d,"
+             "\"waves\":%u,\"ms\":%.3f}",
+             rep.size() > 1 ? "," : "", cd.wl, cd.wpg, cd.gpc, cd.stack, cd.minp, cd.check, cd.age, r.num_waves,
              r.t_count_ms);
     rep += buf;
     if (r.t_count_ms < best_ms) {
@@ -568,8 +573,11 @@ int mce_autotune(mce_graph *h, const mce_opts *base, double sample_fraction, mce
     }
     return MCE_OK;
   };
-  for (const Cand &cd : grid)
-    if ((rc = run(cd))) return rc;
+  for (int v : variants)
+    for (const auto &sh : shapes) {
+      Cand cd{v, sh[0], sh[1], 0, 0, 0, 0};
+      if ((rc = run(cd))) return rc;
+    }
   // then the frame levels a subproblem asks the pool for, then (worker list)
   // the donation thresholds
   const Cand shape = bestc;
@@ -578,7 +586,7 @@ int mce_autotune(mce_graph *h, const mce_opts *base, double sample_fraction, mce
     cd.stack = st;
     if ((rc = run(cd))) return rc;
   }
-  if (base->worklist == MCE_WL) {
+  if (bestc.wl == MCE_WL) {
     const Cand b2 = bestc;
     for (int mp : {4, 10, 24})
       for (int age : {100, 1000, 10000}) {
@@ -590,6 +598,7 @@ int mce_autotune(mce_graph *h, const mce_opts *base, double sample_fraction, mce
   }
   rep += "]";
   *best = *base;
+  best->worklist = bestc.wl;
   best->lds_arena_bytes = 0;
   best->waves_per_group = bestc.wpg;
   best->groups_per_cu = bestc.gpc;

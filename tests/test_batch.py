@@ -80,5 +80,5 @@ def test_autotune_is_result_invariant(mce, oracle):
         fastest = min(trials, key=lambda t: t["ms"])
         for k in ("waves_per_group", "groups_per_cu", "stack_lds_levels"):
             assert best[k] == fastest[k]
-        r = dg.count(level=1, induced=2, worklist=2, **best)
+        r = dg.count(level=1, induced=2, **best)
         assert (r["count"], r["hash"], r["hist"]) == (ref["count"], ref["hash"], ref["hist"])
