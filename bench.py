@@ -99,10 +99,10 @@ def sample_workload(mce, g, off, frac):
     return g, order, f"every {k}th first-level vertex ({order.size} of {g.n})"
 
 
-def measured_traffic(workload_key):
-    """HBM bytes of one enumeration launch from the committed counter summary
-    (profiles/bk_kernel_sbm4096_traffic.txt) when it covers this workload."""
-    path = os.path.join(ROOT, "profiles", "bk_kernel_sbm4096_traffic.txt")
+def counter_summary(name, workload_key):
+    """key = value lines of a committed rocprofv3 counter summary under
+    profiles/, when it was taken on this workload."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     kv = {}
@@ -110,9 +110,7 @@ def measured_traffic(workload_key):
         if "=" in line and not line.startswith("#"):
             k, v = line.split("=", 1)
             kv[k.strip()] = v.strip()
-    if kv.get("workload") != workload_key:
-        return None
-    return float(kv["traffic_bytes"])
+    return kv if kv.get("workload") == workload_key else None
 
 
 def algorithmic_bytes(g, split, ipx=True):
@@ -239,10 +237,20 @@ def main():
     algo_b = algorithmic_bytes(g, split, ipx=args.induced == 2)
     kavg = float(np.mean(kern_ms))
     achieved = algo_b / (kavg / 1e3) / 1e9
-    traffic = None
+    traffic, valu = None, None
     if args.config == "c4":
-        traffic = measured_traffic(f"sbm-batch graphs=4096 n=64..4096 blocks=4 p_in=0.3 p_out=0.01 seed={args.seed} "
-                                   f"level={args.level} induced={args.induced}")
+        key = (f"sbm-batch graphs=4096 n=64..4096 blocks=4 p_in=0.3 p_out=0.01 seed={args.seed} "
+               f"level={args.level} induced={args.induced}")
+        t = counter_summary("bk_kernel_sbm4096_traffic.txt", key)
+        traffic = float(t["traffic_bytes"]) if t else None
+        v = counter_summary("bk_kernel_sbm4096_valu_lds.txt", key)
+        if v:
+            # the kernel's binding resource: VALU issue (bitwise set algebra)
+            valu = {"achieved": float(v["valu_rate_ginst_s"]), "peak": float(v["valu_peak_ginst_s"]),
+                    "unit": "G wave64 VALU inst/s", "frac": float(v["valu_issue_frac"]),
+                    "busy_frac": float(v["valu_busy_frac"]), "insts_per_node": float(v["valu_insts_per_node"]),
+                    "lds_bank_conflict_frac": float(v["lds_bank_conflict_frac"]),
+                    "source": "profiles/bk_kernel_sbm4096_valu_lds.txt"}
     per_graph = None
     if off is not None:
         sc = first["sub_cliques"].astype(np.int64)
@@ -262,7 +270,7 @@ def main():
         "autotune": tuned,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes": algo_b, "kernel": "bk_kernel (enumeration)"},
+                     "algorithmic_bytes": algo_b, "kernel": "bk_kernel (enumeration)", "valu": valu},
     }
     if not args.no_cpu:
         oracle = _import("mce_oracle", os.path.join(ROOT, "oracle", "oracle.py"))
