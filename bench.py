@@ -58,15 +58,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_workload(mce, name, seed):
+def make_workload(mce, name, seed, cl_scale=22):
     """Returns (graph, vtx_off or None, description dict)."""
     if name == "c1":
         return mce.gen_er(4096, 65536, seed), None, dict(workload="er", n=4096, m=65536, seed=seed)
     if name == "c2":
         return mce.gen_rmat(20, 16, seed=seed), None, dict(workload="rmat", scale=20, edge_factor=16, seed=seed)
     if name == "c3":
-        return (mce.gen_chung_lu(1 << 22, 2.1, 24.0, 2000, 20, 60, seed=seed), None,
-                dict(workload="chung-lu", n=1 << 22, beta=2.1, mean_degree=24, planted=2000, k=[20, 60], seed=seed))
+        # the 4 M-vertex configuration plants 2,000 cliques; smaller scales keep n / 2048 of them
+        n = 1 << cl_scale
+        planted = 2000 if cl_scale == 22 else n >> 11
+        return (mce.gen_chung_lu(n, 2.1, 24.0, planted, 20, 60, seed=seed), None,
+                dict(workload="chung-lu", n=n, beta=2.1, mean_degree=24, planted=planted, k=[20, 60], seed=seed))
     if name == "c4":
         off, g = mce.gen_sbm_batch(4096, 64, 4096, 4, 0.3, 0.01, seed=seed)
         return g, off, dict(workload="sbm-batch", graphs=4096, n=[64, 4096], blocks=4, p_in=0.3, p_out=0.01,
@@ -141,9 +144,13 @@ def main():
     ap.add_argument("--induced", type=int, default=2)
     ap.add_argument("--worklist", type=int, default=3, help="1 nowl, 2 wl, 3 auto (picked by the cost model)")
     ap.add_argument("--schedule", choices=["learned", "plain"], default="learned")
-    ap.add_argument("--model", default=os.path.join(ROOT, "mce-gpu_amd", "cost_model_sbm.mlp"))
+    ap.add_argument("--model", default=None, help="cost model file (default: the committed model of the family)")
+    ap.add_argument("--cl-scale", type=int, default=22, help="c3: log2 of the vertex count")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-schedule comparison step")
-    ap.add_argument("--no-autotune", action="store_true", help="keep the default block settings")
+    ap.add_argument("--autotune", choices=["auto", "on", "off"], default="auto",
+                    help="tune block settings before timing (auto: for batches, whose many small subproblems a "
+                         "sample represents well; a sample misjudges a heavy tail)")
+    ap.add_argument("--no-autotune", action="store_true", help="same as --autotune off")
     ap.add_argument("--tune-sample", type=float, default=1.0 / 64,
                     help="fraction of first-level subproblems the autotuner times each trial on")
     ap.add_argument("--cpu-sample", type=float, default=1.0 / 64,
@@ -171,7 +178,10 @@ def main():
     if mce.device_count() < 1:
         raise SystemExit("no GPU visible to libmce.so")
     t0 = time.time()
-    g, off, desc = make_workload(mce, args.config, args.seed)
+    g, off, desc = make_workload(mce, args.config, args.seed, args.cl_scale)
+    if args.model is None:
+        fam = {"c3": "cl", "c4": "sbm"}.get(args.config, "sbm")
+        args.model = os.path.join(ROOT, "mce-gpu_amd", f"cost_model_{fam}.mlp")
     log(f"[bench] generated {args.config}: n={g.n} arcs={g.m} in {time.time() - t0:.1f}s")
 
     model_path = args.model if args.schedule == "learned" else None
@@ -184,7 +194,8 @@ def main():
             log(f"[bench] warmup {w}: count={r['count']} order={info['t_order_ms']:.1f}ms "
                 f"count={r['t_count_ms']:.1f}ms")
         tuned = None
-        if not args.no_autotune:
+        do_tune = args.autotune == "on" or (args.autotune == "auto" and off is not None)
+        if do_tune and not args.no_autotune:
             # block settings (and, left to it, the variant) timed on a sample
             # of this graph; untimed set-up like the warmup
             t = time.perf_counter()

@@ -13,7 +13,8 @@ size), and written in the engine's model format.
 
 Families: sbm (batches of SBM graphs, n in [64, 4096], 4 blocks, p_in .3,
 p_out .01; --graphs per batch) and chung-lu (beta 2.1, mean degree 24,
-planted cliques of 20..60 vertices; --n and --planted).
+planted cliques of 20..60 vertices, n / 2048 of them as in the 4 M-vertex
+configuration; --n and --planted).
 """
 import argparse
 import importlib.util
@@ -41,7 +42,7 @@ def make_graph(mce, family, seed, args):
     if family == "sbm":
         _, g = mce.gen_sbm_batch(args.graphs, 64, 4096, 4, 0.3, 0.01, seed=seed)
         return g
-    return mce.gen_chung_lu(args.n, 2.1, 24.0, args.planted, 20, 60, seed=seed)
+    return mce.gen_chung_lu(args.n, 2.1, 24.0, args.planted or args.n // 2048, 20, 60, seed=seed)
 
 
 def labeled(mce, g, worklist, device=0):
@@ -49,7 +50,8 @@ def labeled(mce, g, worklist, device=0):
     with mce.DeviceGraph(g, device=device) as dg:
         dg.order()
         F = dg.features()
-        r = dg.count(level=mce.L1, induced=mce.IPX, worklist=worklist, sub_nodes=True)
+        # long-running tasks only give work away: labels do not depend on it
+        r = dg.count(level=mce.L1, induced=mce.IPX, worklist=worklist, sub_nodes=True, donate_age_us=10000)
     keep = F[:, 1] > 0  # out-degree >= 1: vertices that root a subproblem
     y = np.log1p(r["sub_nodes"].astype(np.float64)).astype(np.float32)
     return F[keep], y[keep], r
@@ -80,8 +82,8 @@ def main():
     ap.add_argument("--train-seeds", default="2-9")
     ap.add_argument("--eval-seed", type=int, default=1)
     ap.add_argument("--graphs", type=int, default=256, help="sbm: graphs per batch")
-    ap.add_argument("--n", type=int, default=1 << 16, help="chung-lu: vertices")
-    ap.add_argument("--planted", type=int, default=32, help="chung-lu: planted cliques")
+    ap.add_argument("--n", type=int, default=1 << 18, help="chung-lu: vertices")
+    ap.add_argument("--planted", type=int, default=0, help="chung-lu: planted cliques (default n / 2048)")
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=3e-3)
