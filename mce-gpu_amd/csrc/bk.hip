@@ -1012,7 +1012,39 @@ __device__ __forceinline__ void fill_hash_terms(const Sub &s) {
 
 // Builds A over S and the outside-X rows of the q vertices in `ids`; returns
 // the initial outside-X bitmap word of this lane (rows with a neighbour in S).
-__device__ __forceinline__ uint64_t build_bits(const BkParams &pr, Sub &s, const uint32_t *ids) {
+// The outside rows worth keeping: those of the q vertices in `ids` with a
+// neighbour in S (sorted, p ids; rows are N+(x) & S since S lies later than
+// every x). Lane k returns the keep mask of rows 64k..64k+63 (q <= 4096);
+// nkeep is their count. Rows with no bit are never candidates, so dropping
+// them (in order) changes nothing but the memory a subproblem needs.
+__device__ __forceinline__ uint64_t live_rows(const BkParams &pr, const uint32_t *S, uint32_t p, const uint32_t *ids,
+                                              uint32_t q, uint32_t &nkeep) {
+  const uint32_t L = lane_id();
+  uint64_t keep = 0;
+  nkeep = 0;
+  for (uint32_t b = 0, k = 0; b < q; b += kWave, ++k) {
+    const uint32_t i = b + L;
+    bool any = false;
+    if (i < q) {
+      const uint32_t x = ids[i];
+      for (uint64_t t = pr.split[x], e = pr.rp[x + 1]; t < e && !any; ++t) {
+        const uint32_t y = pr.oc[t];
+        const uint32_t j = lower_bound_u32(S, p, y);
+        any = j < p && S[j] == y;
+      }
+    }
+    const uint64_t m = ballot(any);
+    if (L == k) keep = m;
+    nkeep += popc64(m);
+  }
+  return keep;
+}
+
+// Builds A over S and the kept outside rows (keep: mask from live_rows over
+// the q0 vertices in `ids`; s.q = their count), numbered in order; returns
+// this lane's word of the initial outside-row set (all kept rows).
+__device__ __forceinline__ uint64_t build_bits(const BkParams &pr, Sub &s, const uint32_t *ids, uint32_t q0,
+                                               uint64_t keep) {
   const uint32_t L = lane_id(), p = s.p, W = s.W, q = s.q;
   zero_words(s.A, (uint64_t)p * W);
   zero_words(s.XR, (uint64_t)q * W);
@@ -1032,28 +1064,23 @@ __device__ __forceinline__ uint64_t build_bits(const BkParams &pr, Sub &s, const
       }
     }
   }
-  uint64_t E0 = 0;
-  for (uint32_t b = 0, k = 0; b < q; b += kWave, ++k) {
-    const uint32_t i = b + L;
-    bool any = false;
-    if (i < q) {
-      const uint32_t x = ids[i];
-      uint64_t *row = s.XR + (uint64_t)i * W;
+  uint32_t base = 0;
+  for (uint32_t b = 0, k = 0; b < q0; b += kWave, ++k) {
+    const uint64_t km = rdlane64(keep, k);
+    if ((km >> L) & 1ull) {
+      const uint32_t x = ids[b + L];
+      uint64_t *row = s.XR + (uint64_t)(base + mbcnt(km)) * W;
       for (uint64_t t = pr.split[x]; t < pr.rp[x + 1]; ++t) {
         const uint32_t y = pr.oc[t];
         const uint32_t j = lower_bound_u32(s.S, p, y);
-        if (j < p && s.S[j] == y) {
-          row[j >> 6] |= 1ull << (j & 63);
-          any = true;
-        }
+        if (j < p && s.S[j] == y) row[j >> 6] |= 1ull << (j & 63);
       }
     }
-    const uint64_t m = ballot(any);
-    if (L == k) E0 = m;
+    base += popc64(km);
   }
   fill_hash_terms(s);
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-  return E0;
+  return full_word(q);
 }
 
 // Pivot over P|X (index order) then the outside rows in E (row order): the
@@ -1465,11 +1492,12 @@ __device__ __forceinline__ void finish_sub(const BkParams &pr, Sub &s, uint64_t 
 }
 
 template <bool kWL, int kStat>
-__device__ __forceinline__ void finish_bits(const BkParams &pr, Sub &s, const uint32_t *ids,
-                                            unsigned long long *hist_lds, WaveAcc &wa, Clock<kStat> &clk) {
+__device__ __forceinline__ void finish_bits(const BkParams &pr, Sub &s, const uint32_t *ids, uint32_t q0,
+                                            uint64_t keep, unsigned long long *hist_lds, WaveAcc &wa,
+                                            Clock<kStat> &clk) {
   const uint64_t nodes0 = wa.nodes;
   clk.to(kBdInduced);
-  const uint64_t E0 = build_bits(pr, s, ids);
+  const uint64_t E0 = build_bits(pr, s, ids, q0, keep);
   solve_bits<kWL, kStat>(pr, s, full_word(s.p), 0, E0, hist_lds, wa, clk);
   clk.to(kBdOther);
   if (lane_id() == 0) {
@@ -1537,25 +1565,28 @@ __global__ __launch_bounds__(kMaxWpg * kWave, 4) void bk_kernel(BkParams pr) {
         if (p == 0) continue;
         const uint32_t W = (p + 63) / 64;
         const bool bits = kIPX && q <= kMaxXRows;
-        const Region rg = bits ? pool_take_wait(pl, bits_want(p, q, W, lv), bits_want_inner(p, q, W, lv),
-                                                bits_fs(W, (q + 63) / 64) * min(p, lv), bits_want_inner(p, q, W, lv))
+        uint32_t qb = q;
+        const uint64_t keep = bits ? live_rows(pr, pr.oc + s0, p, pr.oc + b0, q, qb) : 0ull;
+        const Region rg = bits ? pool_take_wait(pl, bits_want(p, qb, W, lv), bits_want_inner(p, qb, W, lv),
+                                                bits_fs(W, (qb + 63) / 64) * min(p, lv),
+                                                bits_want_inner(p, qb, W, lv))
                                : pool_take_wait(pl, list_want(p, q, W, kIPX, lv), 0, (16 * W + 16) * min(p, lv),
                                                 list_want(p, q, W, kIPX, lv));
-        if (!(bits ? place_bits(s, p, q, W, rg.ptr, rg.bytes, hbm, pr.hbm_bytes, pr.stack_lds)
+        if (!(bits ? place_bits(s, p, qb, W, rg.ptr, rg.bytes, hbm, pr.hbm_bytes, pr.stack_lds)
                    : place(s, p, q, W, kIPX, rg.ptr, rg.bytes, hbm, pr.hbm_bytes, pr.stack_lds))) {
           pool_put(pl, rg);
           defer(pr, it);
           continue;
         }
         s.p = p;
-        s.q = q;
+        s.q = bits ? qb : q;
         s.W = W;
         s.rsize = 1;
         s.root = v;
         s.rmix = splitmix64(v);
         for (uint32_t i = L; i < p; i += kWave) s.S[i] = pr.oc[s0 + i];
         if (bits) {
-          finish_bits<kWL, kStat>(pr, s, pr.oc + b0, hist_lds, wa, clk);
+          finish_bits<kWL, kStat>(pr, s, pr.oc + b0, q, keep, hist_lds, wa, clk);
         } else {
           for (uint32_t i = L; i < q; i += kWave) s.XL[i] = pr.oc[b0 + i];
           finish_sub<kIPX, kWL, kStat>(pr, s, full_word(p), 0, hist_lds, wa, clk);
@@ -1585,10 +1616,12 @@ __global__ __launch_bounds__(kMaxWpg * kWave, 4) void bk_kernel(BkParams pr) {
           pool_put(pl, rg);
           continue;
         }
-        // re-lay the arena for the actual sizes
+        // re-lay the arena for the actual sizes (and, bitmap path, the kept rows)
         const uint32_t W = (p + 63) / 64;
         const bool bits = kIPX && q <= kMaxXRows;
-        if (!(bits ? place_bits(s, p, q, W, rg.ptr, rg.bytes, hbm, pr.hbm_bytes, pr.stack_lds)
+        uint32_t qk = q;
+        const uint64_t keep = bits ? live_rows(pr, t.S, p, t.XL, q, qk) : 0ull;
+        if (!(bits ? place_bits(s, p, qk, W, rg.ptr, rg.bytes, hbm, pr.hbm_bytes, pr.stack_lds)
                    : place(s, p, q, W, kIPX, rg.ptr, rg.bytes, hbm, pr.hbm_bytes, pr.stack_lds))) {
           pool_put(pl, rg);
           defer(pr, it);
@@ -1599,12 +1632,12 @@ __global__ __launch_bounds__(kMaxWpg * kWave, 4) void bk_kernel(BkParams pr) {
         if (s.S != t.S) wave_copy(s.S, t.S, p);
         if (!bits && s.XL != t.XL) wave_copy(s.XL, t.XL, q);
         s.p = p;
-        s.q = q;
+        s.q = bits ? qk : q;
         s.W = W;
         s.rsize = 2;
         s.root = u;
         s.rmix = rmix;
-        if (bits) finish_bits<kWL, kStat>(pr, s, t.XL, hist_lds, wa, clk);
+        if (bits) finish_bits<kWL, kStat>(pr, s, t.XL, q, keep, hist_lds, wa, clk);
         else finish_sub<kIPX, kWL, kStat>(pr, s, full_word(p), 0, hist_lds, wa, clk);
         pool_put(pl, rg);
       }
