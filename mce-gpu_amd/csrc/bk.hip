@@ -20,24 +20,25 @@
 // and the branch mask live one word per lane (lane k holds word k) in
 // registers, so intersections, emptiness tests and next-candidate search are
 // single wave instructions plus a ballot, with no workgroup barriers on the
-// hot loop. Pivot scores run one candidate row per lane. The per-wave arena
-// (S, the matrix, the outside-X rows and lists) sits in LDS when it fits the
-// bitmap cutoff and in a per-wave HBM arena otherwise; stack frames beyond
-// the LDS budget spill to HBM. Tie-breaks are deterministic (lowest index,
-// outside-X rows by vertex id), so the tree is reproducible, matches the CPU
-// oracle node for node, and a donated subtree expands exactly as it would
-// have in the donor.
+// hot loop. Pivot scores run one candidate row per lane. A subproblem's arena
+// (S, the matrix, the outside-X rows and lists, the hash terms, the first
+// stack frames) comes from its workgroup's LDS pool when the pool has room and
+// from the wave's HBM arena otherwise; deeper frames spill to HBM. Tie-breaks
+// are deterministic (lowest index, outside-X rows by vertex id), so the tree
+// is reproducible, matches the CPU oracle node for node, and a donated
+// subtree expands exactly as it would have in the donor.
 //
-// Worker list: a bounded lock-free ring of task slots in HBM (per-slot
-// sequence numbers, agent-scope acquire/release so a slot written on one XCD
-// is seen on another). When idle waves outnumber queued tasks, a busy wave
-// takes the next candidate of its shallowest frame (the largest remaining
-// subtree) and publishes that child: its P and X words, the donor's vertex
-// set and bit matrix, and the outside-X entries that survive. The receiver
-// copies them into its own arena and continues in the donor's index space,
-// so no arena is ever shared and nothing is rebuilt. A counter of outstanding work
-// (tickets + published tasks) ends the kernel exactly; a realtime watchdog
-// turns any termination bug into an error flag instead of a hang.
+// Worker list: an idle wave puts its id on a bounded lock-free queue in HBM
+// (per-slot sequence numbers, agent-scope acquire/release so that a slot
+// written on one XCD is seen on another) and waits on its own mailbox line.
+// A busy wave whose task has run long enough takes an idle wave off the queue
+// and writes into that wave's task slot the next candidate of its shallowest
+// frame (the largest remaining subtree): its P and X words, the donor's vertex
+// set and bit matrix, and the outside-X rows that survive. The receiver
+// copies them into its own arena and continues in the donor's index space, so
+// no arena is ever shared and nothing is rebuilt. A counter of outstanding
+// work (tickets + handed-over tasks) ends the kernel exactly; a realtime
+// watchdog turns any termination bug into an error flag instead of a hang.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -66,7 +67,8 @@ enum Acc : uint32_t {
 enum Stat : int { kStatNone = 0, kStatLb = 1, kStatDonor = 2, kStatBd = 3 };
 enum Bd : int { kBdOther = 0, kBdInduced, kBdQueue, kBdPivot, kBdSetOps, kBdMaximal, kBdBranch };
 
-// Ring control words, each on its own 64-byte line.
+// Idle-queue control words (head, tail) and the outstanding-work counter,
+// each on its own 64-byte line.
 constexpr uint32_t kCtlHead = 0, kCtlTail = 8, kCtlOutstanding = 16, kCtlWords = 32;
 // Mailbox words of the waves, each on its own 128-byte line.
 constexpr uint32_t kMboxStride = 16;
@@ -113,7 +115,7 @@ struct BkParams {
   // worker list
   unsigned long long *ctl;
   unsigned long long *seq;
-  uint8_t *ring;
+  uint8_t *ring;           // task slots, one per wave
   uint32_t ring_cap;       // idle-queue capacity (= waves)
   uint32_t *idle_ids;      // idle queue entries: wave ids
   unsigned long long *mbox;  // per-wave mailbox: 1 once a task is in the wave's slot
@@ -614,13 +616,14 @@ __device__ __forceinline__ void build(const BkParams &pr, Sub &s) {
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
 }
 
-// ---- worker-list ring ---------------------------------------------------
+// ---- worker list: idle-wave queue and mailboxes ---------------------------
 
 __device__ __forceinline__ unsigned long long ld_relaxed(unsigned long long *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Claims a slot to publish into (lane 0 only). kNoSlot when full or contended.
+// Claims a queue position to publish into (lane 0 only). kNoSlot when full or
+// contended.
 __device__ __forceinline__ unsigned long long ring_claim_push(const BkParams &pr) {
   unsigned long long *tail = pr.ctl + kCtlTail;
   unsigned long long pos = ld_relaxed(tail);
@@ -641,7 +644,8 @@ __device__ __forceinline__ unsigned long long ring_claim_push(const BkParams &pr
   return kNoSlot;
 }
 
-// Claims a published slot to consume (lane 0 only). kNoSlot when empty.
+// Claims a published queue position to consume (lane 0 only). kNoSlot when
+// empty or contended.
 __device__ __forceinline__ unsigned long long ring_claim_pop(const BkParams &pr) {
   unsigned long long *head = pr.ctl + kCtlHead;
   unsigned long long pos = ld_relaxed(head);
@@ -662,8 +666,7 @@ __device__ __forceinline__ unsigned long long ring_claim_pop(const BkParams &pr)
   return kNoSlot;
 }
 
-// Idle waves minus queued tasks, sampled by lane 0.
-// Idle waves waiting for work: the length of the idle queue.
+// Idle waves waiting for work: the length of the idle queue (lane 0 samples).
 __device__ __forceinline__ int32_t donation_budget(const BkParams &pr) {
   int32_t b = 0;
   if (lane_id() == 0) {
@@ -2071,7 +2074,7 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   MCE_HIP(hipMemsetAsync(ticket, 0, sizeof(uint32_t), s));
   if (cfg.capture) {
     // one persistent launch, captured into a graph: counters reset by a kernel
-    // (no memset nodes), then the ring set-up and the enumeration kernel
+    // (no memset nodes), then the worker-list set-up and the enumeration kernel
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     MCE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
