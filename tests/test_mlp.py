@@ -163,3 +163,30 @@ def test_features_match_numpy(mce, oracle):
         assert abs(F[v, 7] - np.log1p(eP)) < 1e-4
         if p > 1:
             assert abs(F[v, 8] - 2 * eP / (p * (p - 1))) < 1e-5
+
+
+@pytest.mark.gpu
+def test_auto_variant_is_result_invariant(mce, oracle):
+    """-w auto: the committed SBM model predicts no heavy tail on an SBM batch
+    (no worker list, so no donations); any pick leaves the results alone."""
+    import os
+    model = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "mce-gpu_amd",
+                         "cost_model_sbm.mlp")
+    off, g = mce.gen_sbm_batch(48, n_min=64, n_max=700, blocks=4, p_in=0.3, p_out=0.01, seed=21)
+    ref = oracle.mce(g, level=1, ipx=True)
+    with mce.DeviceGraph(g) as dg:
+        dg.order()
+        r = dg.count(level=1, induced=2, worklist=mce.WL_AUTO, model_path=model)
+        assert r["fallback_events"] == 0 and r["donations"] == 0
+        assert (r["count"], r["hash"], r["hist"], r["tree_nodes"]) == (
+            ref["count"], ref["hash"], ref["hist"], ref["tree_nodes"])
+        # the plain schedule under auto takes the worker list
+        r = dg.count(level=1, induced=2, worklist=mce.WL_AUTO)
+        assert (r["count"], r["hash"]) == (ref["count"], ref["hash"])
+    g = mce.gen_rmat(12, 16, seed=13)
+    ref = oracle.mce(g, level=1, ipx=True)
+    with mce.DeviceGraph(g) as dg:
+        dg.order()
+        r = dg.count(level=1, induced=2, worklist=mce.WL_AUTO, model_path=model)
+        assert (r["count"], r["hash"], r["hist"], r["tree_nodes"]) == (
+            ref["count"], ref["hash"], ref["hist"], ref["tree_nodes"])
