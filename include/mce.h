@@ -107,6 +107,14 @@ typedef struct mce_opts {
   int donate_age_us;       /* a task donates only after running this long
                               (0 = 1000 us): short tasks finish sooner
                               than a copy of their bit matrix pays back    */
+  int split_heavy;         /* learned schedule at level 1, the variant per
+                              subproblem: N > 0 runs every vertex predicted
+                              above N tree nodes as its level-2 arc
+                              subproblems; -1 picks the cut (1/8 of a
+                              wave's share of the work, at least 64 K
+                              nodes); 0 = never (default: on the heavy-
+                              tailed graphs measured, the unpivoted arc
+                              roots cost more than they balance)            */
 
   int schedule;            /* enum mce_schedule                              */
   const uint32_t *order;   /* MCE_SCHED_GIVEN: first-level vertex order      */
@@ -141,6 +149,7 @@ typedef struct mce_result {
   uint64_t tree_nodes;       /* BK tree nodes expanded                  */
   uint64_t donations;        /* worker-list donations                   */
   uint64_t fallback_events;  /* learned schedule fell back to plain     */
+  uint64_t split_vertices;   /* vertices run as level-2 arc subproblems */
   double t_order_ms;         /* k-core ordering, device time            */
   double t_orient_ms;        /* orientation + degree maxima, device time */
   double t_count_ms;         /* enumeration, device time                */

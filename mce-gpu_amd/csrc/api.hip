@@ -9,6 +9,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -85,6 +86,7 @@ int check_opts(const mce_opts *o) {
     return set_error(MCE_EINVAL, "negative block setting");
   if (o->lds_arena_bytes && (o->lds_arena_bytes < 1024 || o->lds_arena_bytes % 8))
     return set_error(MCE_EINVAL, "lds_arena_bytes must be a multiple of 8 and >= 1024");
+  if (o->split_heavy < -1) return set_error(MCE_EINVAL, "split_heavy must be -1, 0 or a node count");
   if (o->schedule < MCE_SCHED_PLAIN || o->schedule > MCE_SCHED_GIVEN) return set_error(MCE_EINVAL, "unknown schedule");
   if (o->schedule == MCE_SCHED_GIVEN && !o->order) return set_error(MCE_EINVAL, "given schedule without an order");
   return MCE_OK;
@@ -233,6 +235,19 @@ int mce_graph_download(const mce_graph *h, uint32_t *priority, uint64_t *split, 
   return MCE_OK;
 }
 
+// Flags the vertices whose cost-model key reaches kmin (their subtrees are
+// split into level-2 arc subproblems) and counts them.
+__global__ void mark_heavy(uint32_t n, const uint32_t *__restrict__ keys, uint32_t kmin, uint8_t *__restrict__ flag,
+                           unsigned long long *count) {
+  uint32_t c = 0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const bool h = keys[v] >= kmin;
+    flag[v] = h;
+    c += h;
+  }
+  if (c) atomicAdd(count, (unsigned long long)c);
+}
+
 static int count_impl(mce_graph *h, const mce_opts *opts, mce_result *res, bool capture);
 
 int mce_graph_count(mce_graph *h, const mce_opts *opts, mce_result *res) {
@@ -252,6 +267,7 @@ static int count_impl(mce_graph *h, const mce_opts *opts, mce_result *res, bool 
   hipStream_t s = h->s;
   fill_order_fields(h, res);
   res->count = res->hash = res->tree_nodes = res->donations = res->fallback_events = 0;
+  res->split_vertices = 0;
   uint64_t fallback = 0;
   res->max_clique_size = 0;
   res->wave_stats_len = 0;
@@ -261,7 +277,11 @@ static int count_impl(mce_graph *h, const mce_opts *opts, mce_result *res, bool 
   uint32_t *order = nullptr;
   uint64_t *hist = nullptr, *sub_nodes = nullptr, *sub_cliques = nullptr;
   uint2 *arcs = nullptr;
+  uint8_t *split_flag = nullptr;
+  unsigned long long *n_split = nullptr;
   auto cleanup = [&]() {
+    hipFree(split_flag);
+    hipFree(n_split);
     hipFree(order);
     hipFree(hist);
     hipFree(sub_nodes);
@@ -311,8 +331,25 @@ static int count_impl(mce_graph *h, const mce_opts *opts, mce_result *res, bool 
             rc = schedule_by_key(g, keys, 4095, order, &len, s);
             learned = rc == MCE_OK;
             learned_used = learned;
-            // a predicted subtree larger than a wave's share: donate early and small
-            if (learned && wmax * 4096.0 > wsum) heavy_tail = true;
+            // a predicted subtree larger than a wave's share of the work (16
+            // waves per CU): a heavy tail
+            int dev = 0, cus = 256;
+            if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            const double share = wsum / (16.0 * std::max(cus, 1));
+            if (learned && wmax > share) heavy_tail = true;
+            // the variant per subproblem (when asked for): a vertex predicted
+            // above the cut runs as its level-2 arc subproblems, the rest
+            // whole; the automatic cut is 1/8 of a share and at least 64 K
+            // nodes (smaller trees do not repay the per-arc rebuilds)
+            const double cut = opts->split_heavy > 0 ? (double)opts->split_heavy : std::max(share / 8, 65536.0);
+            if (learned && opts->level == MCE_L1 && opts->split_heavy != 0 && wmax > cut && share > 0) {
+              const double kt = std::ceil(64.0 * std::log1p(cut));
+              if (kt <= 4095.0 && hipMalloc(&split_flag, g.n) == hipSuccess &&
+                  hipMalloc(&n_split, sizeof(unsigned long long)) == hipSuccess) {
+                MCE_HIP(hipMemsetAsync(n_split, 0, sizeof(unsigned long long), s));
+                mark_heavy<<<1024, 256, 0, s>>>(g.n, keys, (uint32_t)kt, split_flag, n_split);
+              }
+            }
           }
           hipFree(keys);
         }
@@ -331,13 +368,20 @@ static int count_impl(mce_graph *h, const mce_opts *opts, mce_result *res, bool 
   }
   const void *items = order;
   uint64_t n_items = len;
-  if (opts->level == MCE_L2) {
-    rc = arcs_from_order(g, order, len, &arcs, &n_items, s);
+  int level = opts->level;
+  uint64_t split_vertices = 0;
+  if (split_flag && n_split) {
+    MCE_HIP(hipMemcpyAsync(&split_vertices, n_split, sizeof(split_vertices), hipMemcpyDeviceToHost, s));
+    MCE_HIP(hipStreamSynchronize(s));
+  }
+  if (opts->level == MCE_L2 || split_vertices) {
+    rc = arcs_from_order(g, order, len, opts->level == MCE_L2 ? nullptr : split_flag, &arcs, &n_items, s);
     if (rc) {
       cleanup();
       return rc;
     }
     items = arcs;
+    if (opts->level == MCE_L1) level = 3;
   }
   if (hipMalloc(&hist, sizeof(uint64_t) * kHistLen) != hipSuccess) {
     cleanup();
@@ -353,7 +397,7 @@ static int count_impl(mce_graph *h, const mce_opts *opts, mce_result *res, bool 
     MCE_HIP(hipMemsetAsync(sub_cliques, 0, sizeof(uint64_t) * g.n, s));
   }
   BkLaunch cfg;
-  cfg.level = opts->level;
+  cfg.level = level;
   cfg.ipx = opts->induced == MCE_IPX;
   // the variant: a given one, or (auto) the worker list only where the cost
   // model predicts a heavy tail
@@ -395,6 +439,7 @@ static int count_impl(mce_graph *h, const mce_opts *opts, mce_result *res, bool 
   if (opts->sub_cliques && g.n)
     MCE_HIP(hipMemcpy(opts->sub_cliques, sub_cliques, sizeof(uint64_t) * g.n, hipMemcpyDeviceToHost));
   res->fallback_events = fallback;
+  res->split_vertices = split_vertices;
   res->count = out.count;
   res->hash = out.hash;
   res->tree_nodes = out.nodes;

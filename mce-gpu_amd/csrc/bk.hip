@@ -1561,8 +1561,24 @@ __global__ __launch_bounds__(kMaxWpg * kWave, 4) void bk_kernel(BkParams pr) {
     for (uint64_t it = lo; it < hi; ++it) {
       wa.sub_cliques = 0;
       wa.t_task = __builtin_amdgcn_s_memrealtime();
-      if (pr.level == 1) {
-        const uint32_t v = uniform(items1[it]);
+      // level 1: vertex items; level 2: arc items; level 3: arc items where
+      // (v, kNone) stands for the whole vertex subproblem of v
+      bool vertex_item = pr.level == 1;
+      uint32_t iv = 0;
+      uint2 ia = make_uint2(0, 0);
+      if (vertex_item) {
+        iv = uniform(items1[it]);
+      } else {
+        ia = items2[it];
+        ia.x = uniform(ia.x);
+        ia.y = uniform(ia.y);
+        if (pr.level == 3 && ia.y == kNone) {
+          vertex_item = true;
+          iv = ia.x;
+        }
+      }
+      if (vertex_item) {
+        const uint32_t v = iv;
         const uint64_t b0 = pr.rp[v], s0 = pr.split[v], e0 = pr.rp[v + 1];
         const uint32_t p = (uint32_t)(e0 - s0), q = (uint32_t)(s0 - b0);
         if (p == 0) continue;
@@ -1596,8 +1612,7 @@ __global__ __launch_bounds__(kMaxWpg * kWave, 4) void bk_kernel(BkParams pr) {
         }
         pool_put(pl, rg);
       } else {
-        const uint2 a = items2[it];
-        const uint32_t u = uniform(a.x), w = uniform(a.y);
+        const uint32_t u = ia.x, w = ia.y;
         const uint64_t su = pr.split[u], eu = pr.rp[u + 1], bu = pr.rp[u];
         const uint64_t sw = pr.split[w], ew = pr.rp[w + 1], bw = pr.rp[w];
         const uint32_t pb = (uint32_t)min(eu - su, ew - sw), qb = (uint32_t)min(eu - bu, sw - bw);
@@ -1838,20 +1853,27 @@ __global__ void need_kernel(const uint64_t *rp, const uint64_t *split, const voi
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_items; i += stride) {
     uint32_t p, q;
+    bool arc = false;
     if (level == 1) {
       const uint32_t v = ((const uint32_t *)items)[i];
       p = (uint32_t)(rp[v + 1] - split[v]);
       q = (uint32_t)(split[v] - rp[v]);
     } else {
       const uint2 a = ((const uint2 *)items)[i];
-      p = (uint32_t)min(rp[a.x + 1] - split[a.x], rp[a.y + 1] - split[a.y]);
-      q = (uint32_t)min(rp[a.x + 1] - rp[a.x], split[a.y] - rp[a.y]);
+      if (level == 3 && a.y == kNone) {
+        p = (uint32_t)(rp[a.x + 1] - split[a.x]);
+        q = (uint32_t)(split[a.x] - rp[a.x]);
+      } else {
+        arc = true;
+        p = (uint32_t)min(rp[a.x + 1] - split[a.x], rp[a.y + 1] - split[a.y]);
+        q = (uint32_t)min(rp[a.x + 1] - rp[a.x], split[a.y] - rp[a.y]);
+      }
     }
     // level-2 items stage their intersections in the list layout of the
     // bounds, then re-lay for the actual sizes (up to four more HBM frames)
     uint64_t need;
     // (sizes for a subproblem the LDS pool gives nothing)
-    if (level == 2)
+    if (arc)
       need = max(hbm_need(p, q, ipx), ipx && q <= kMaxXRows ? bits_need(p, q) : 0ull) +
              4ull * (8 * (2 * ((p + 63) / 64) + (q + 63) / 64) + 16);
     else

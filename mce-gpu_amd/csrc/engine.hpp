@@ -43,7 +43,7 @@ int schedule_by_key(const DevGraph &g, const uint32_t *key, uint32_t key_max, ui
                     uint32_t *len_out, hipStream_t s);
 
 struct BkLaunch {
-  int level = 1;       // 1 = vertex subproblems, 2 = arc subproblems
+  int level = 1;       // 1 = vertex subproblems, 2 = arc subproblems, 3 = both (mixed list)
   bool ipx = true;
   bool wl = false;
   int task = 0;        // enum mce_task
@@ -66,9 +66,11 @@ struct BkOut {
 };
 
 // Arc work list for level-2 subproblems: for every vertex u of `order`, its
-// out-arcs (u, w) in order, packed as (u, w) pairs. Returns the arc count.
-int arcs_from_order(const DevGraph &g, const uint32_t *order, uint32_t order_len, uint2 **arcs_out,
-                    uint64_t *n_arcs, hipStream_t s);
+// out-arcs (u, w) in order, packed as (u, w) pairs. With `split_flag`, only
+// flagged vertices are expanded; the others stay whole as (u, 0xFFFFFFFF)
+// (a mixed level-1/level-2 list, launched as level 3). Returns the item count.
+int arcs_from_order(const DevGraph &g, const uint32_t *order, uint32_t order_len, const uint8_t *split_flag,
+                    uint2 **arcs_out, uint64_t *n_arcs, hipStream_t s);
 
 // Enumerates maximal cliques over a first-level work list.
 // level 1: items are vertex ids (uint32); level 2: items are arcs (uint2).

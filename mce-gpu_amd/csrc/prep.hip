@@ -260,12 +260,19 @@ __global__ __launch_bounds__(kBlock) void sched_scatter(uint32_t n, const uint32
 // ---- arc list for level-2 subproblems ------------------------------------
 constexpr uint32_t kScanTile = 1024;
 
+// Items per ordered vertex: its out-degree (one arc item per out-neighbour)
+// when it is split (all vertices when `split_flag` is null), else one.
 __global__ __launch_bounds__(kScanTile) void scan_tiles(uint32_t len, const uint32_t *__restrict__ order,
-                                                        const uint32_t *__restrict__ dplus, uint64_t *off,
+                                                        const uint32_t *__restrict__ dplus,
+                                                        const uint8_t *__restrict__ split_flag, uint64_t *off,
                                                         uint64_t *tile_sums) {
   __shared__ uint64_t s[kScanTile];
   const uint32_t i = blockIdx.x * kScanTile + threadIdx.x;
-  const uint64_t x = i < len ? dplus[order[i]] : 0;
+  uint64_t x = 0;
+  if (i < len) {
+    const uint32_t v = order[i];
+    x = (!split_flag || split_flag[v]) ? dplus[v] : 1;
+  }
   s[threadIdx.x] = x;
   __syncthreads();
   for (uint32_t d = 1; d < kScanTile; d <<= 1) {
@@ -292,6 +299,7 @@ __global__ void scan_sums(uint32_t tiles, uint64_t *tile_sums, uint64_t *total) 
 __global__ __launch_bounds__(kBlock) void write_arcs(uint32_t len, const uint32_t *__restrict__ order,
                                                      const uint64_t *__restrict__ off,
                                                      const uint64_t *__restrict__ tile_sums,
+                                                     const uint8_t *__restrict__ split_flag,
                                                      const uint64_t *__restrict__ split,
                                                      const uint64_t *__restrict__ rp,
                                                      const uint32_t *__restrict__ oc, uint2 *__restrict__ arcs) {
@@ -300,6 +308,10 @@ __global__ __launch_bounds__(kBlock) void write_arcs(uint32_t len, const uint32_
   for (uint32_t i = wid; i < len; i += waves) {
     const uint32_t u = order[i];
     const uint64_t o = off[i] + tile_sums[i / kScanTile];
+    if (split_flag && !split_flag[u]) {
+      if (lane_id() == 0) arcs[o] = make_uint2(u, 0xFFFFFFFFu);  // the whole vertex subproblem
+      continue;
+    }
     const uint64_t b = split[u], e = rp[u + 1];
     for (uint64_t k = b + lane_id(); k < e; k += kWave) arcs[o + (k - b)] = make_uint2(u, oc[k]);
   }
@@ -410,8 +422,8 @@ int schedule_by_key(const DevGraph &g, const uint32_t *key, uint32_t key_max, ui
   return MCE_OK;
 }
 
-int arcs_from_order(const DevGraph &g, const uint32_t *order, uint32_t len, uint2 **arcs_out,
-                    uint64_t *n_arcs, hipStream_t s) {
+int arcs_from_order(const DevGraph &g, const uint32_t *order, uint32_t len, const uint8_t *split_flag,
+                    uint2 **arcs_out, uint64_t *n_arcs, hipStream_t s) {
   *arcs_out = nullptr;
   *n_arcs = 0;
   if (len == 0) return MCE_OK;
@@ -419,7 +431,7 @@ int arcs_from_order(const DevGraph &g, const uint32_t *order, uint32_t len, uint
   uint64_t *off = nullptr, *sums = nullptr;
   MCE_HIP(hipMalloc(&off, sizeof(uint64_t) * len));
   MCE_HIP(hipMalloc(&sums, sizeof(uint64_t) * (tiles + 1)));
-  scan_tiles<<<tiles, kScanTile, 0, s>>>(len, order, g.dplus, off, sums);
+  scan_tiles<<<tiles, kScanTile, 0, s>>>(len, order, g.dplus, split_flag, off, sums);
   scan_sums<<<1, 1, 0, s>>>(tiles, sums, sums + tiles);
   uint64_t total = 0;
   MCE_HIP(hipMemcpyAsync(&total, sums + tiles, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
@@ -427,8 +439,8 @@ int arcs_from_order(const DevGraph &g, const uint32_t *order, uint32_t len, uint
   uint2 *arcs = nullptr;
   if (total) {
     MCE_HIP(hipMalloc(&arcs, sizeof(uint2) * total));
-    write_arcs<<<grid_for((uint64_t)len * kWave, kBlock, 8192), kBlock, 0, s>>>(len, order, off, sums, g.split,
-                                                                                   g.rp, g.oc, arcs);
+    write_arcs<<<grid_for((uint64_t)len * kWave, kBlock, 8192), kBlock, 0, s>>>(len, order, off, sums, split_flag,
+                                                                                   g.split, g.rp, g.oc, arcs);
   }
   MCE_HIP(hipStreamSynchronize(s));
   hipFree(off);

@@ -130,10 +130,16 @@ def test_learned_schedule_is_result_invariant(mce, oracle, tmp_path):
         m = mce.mlp_init(F, seed=1)
         mce.mlp_train(m, F, y, epochs=10, batch=1024, lr=3e-3)
         for wl in (1, 2):
+            # whole vertex subproblems only: the same tree as the oracle's
             r = dg.count(level=1, induced=2, worklist=wl, model=m)
-            assert r["fallback_events"] == 0
+            assert r["fallback_events"] == 0 and r["split_vertices"] == 0
             assert (r["count"], r["hash"], r["hist"], r["tree_nodes"]) == (
                 ref["count"], ref["hash"], ref["hist"], ref["tree_nodes"])
+            # predicted-heavy vertices as level-2 arc subproblems (the cut
+            # picked automatically): other trees, the same cliques
+            r = dg.count(level=1, induced=2, worklist=wl, model=m, split_heavy=-1)
+            assert r["fallback_events"] == 0
+            assert (r["count"], r["hash"], r["hist"]) == (ref["count"], ref["hash"], ref["hist"])
         # a corrupt or missing model falls back to the plain schedule, logged
         bad = tmp_path / "bad.bin"
         bad.write_bytes(b"garbage")
@@ -190,3 +196,23 @@ def test_auto_variant_is_result_invariant(mce, oracle):
         r = dg.count(level=1, induced=2, worklist=mce.WL_AUTO, model_path=model)
         assert (r["count"], r["hash"], r["hist"], r["tree_nodes"]) == (
             ref["count"], ref["hash"], ref["hist"], ref["tree_nodes"])
+
+
+@pytest.mark.gpu
+def test_split_heavy_vertices(mce, oracle):
+    """A model that predicts one vertex far heavier than the rest splits it
+    into its arc subproblems; the per-root clique counts still match."""
+    g = mce.gen_rmat(11, 16, seed=4)
+    ref = oracle.mce(g, level=1, ipx=True, sub_cliques=True)
+    with mce.DeviceGraph(g) as dg:
+        dg.order()
+        F = dg.features()
+        lab = dg.count(level=1, induced=2, sub_nodes=True)
+        y = np.log1p(lab["sub_nodes"].astype(np.float64)).astype(np.float32)
+        m = mce.mlp_init(F, seed=2)
+        mce.mlp_train(m, F, y, epochs=30, batch=512, lr=3e-3)
+        for wl in (1, 2):
+            r = dg.count(level=1, induced=2, worklist=wl, model=m, sub_cliques=True, split_heavy=64)
+            assert r["split_vertices"] > 0
+            assert (r["count"], r["hash"], r["hist"]) == (ref["count"], ref["hash"], ref["hist"])
+            assert np.array_equal(r["sub_cliques"], ref["sub_cliques"])
