@@ -187,6 +187,9 @@ def main():
     model_path = args.model if args.schedule == "learned" else None
     kw = dict(level=args.level, induced=args.induced, worklist=args.worklist,
               schedule=SCHEDULES[args.schedule], model_path=model_path)
+    if off is not None:
+        # the batched engine: one persistent enumeration launch captured into a hipGraph
+        kw["graph_launch"] = 1
     with mce.DeviceGraph(g) as dg:
         for w in range(args.warmup):
             info = dg.order()

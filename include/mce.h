@@ -115,6 +115,8 @@ typedef struct mce_opts {
                               nodes); 0 = never (default: on the heavy-
                               tailed graphs measured, the unpivoted arc
                               roots cost more than they balance)            */
+  int graph_launch;        /* 1 = run the enumeration as one launch captured
+                              into a hipGraph (what mce_run_batch does)     */
 
   int schedule;            /* enum mce_schedule                              */
   const uint32_t *order;   /* MCE_SCHED_GIVEN: first-level vertex order      */

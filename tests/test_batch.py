@@ -82,3 +82,21 @@ def test_autotune_is_result_invariant(mce, oracle):
             assert best[k] == fastest[k]
         r = dg.count(level=1, induced=2, **best)
         assert (r["count"], r["hash"], r["hist"]) == (ref["count"], ref["hash"], ref["hist"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("wl", [1, 2])
+def test_graph_launch_on_resident_graph(mce, oracle, wl):
+    """The captured (hipGraph) enumeration launch on a resident batch, twice in
+    a row with ordinary launches in between, against the oracle."""
+    off, g = mce.gen_sbm_batch(24, n_min=64, n_max=500, blocks=4, p_in=0.3, p_out=0.02, seed=5)
+    ref = oracle.mce(g, level=1, ipx=True, sub_cliques=True)
+    with mce.DeviceGraph(g) as dg:
+        dg.order()
+        for _ in range(2):
+            r = dg.count(level=1, induced=2, worklist=wl, graph_launch=1, sub_cliques=True)
+            assert (r["count"], r["hash"], r["hist"], r["tree_nodes"]) == (
+                ref["count"], ref["hash"], ref["hist"], ref["tree_nodes"])
+            assert np.array_equal(r["sub_cliques"], ref["sub_cliques"])
+            r = dg.count(level=1, induced=2, worklist=wl)
+            assert (r["count"], r["hash"]) == (ref["count"], ref["hash"])
