@@ -1043,6 +1043,9 @@ __device__ __forceinline__ uint64_t live_rows(const BkParams &pr, const uint32_t
   return keep;
 }
 
+// Keep mask of all q rows (lane k: rows 64k..64k+63).
+__device__ __forceinline__ uint64_t all_rows(uint32_t q) { return full_word(q); }
+
 // Builds A over S and the kept outside rows (keep: mask from live_rows over
 // the q0 vertices in `ids`; s.q = their count), numbered in order; returns
 // this lane's word of the initial outside-row set (all kept rows).
@@ -1584,8 +1587,10 @@ __global__ __launch_bounds__(kMaxWpg * kWave, 4) void bk_kernel(BkParams pr) {
         if (p == 0) continue;
         const uint32_t W = (p + 63) / 64;
         const bool bits = kIPX && q <= kMaxXRows;
+        // outside rows: all of them, or (many more than |P|, as at hubs) only
+        // those with a neighbour in P, in order
         uint32_t qb = q;
-        const uint64_t keep = bits ? live_rows(pr, pr.oc + s0, p, pr.oc + b0, q, qb) : 0ull;
+        const uint64_t keep = !bits ? 0ull : q > 2 * p ? live_rows(pr, pr.oc + s0, p, pr.oc + b0, q, qb) : all_rows(q);
         const Region rg = bits ? pool_take_wait(pl, bits_want(p, qb, W, lv), bits_want_inner(p, qb, W, lv),
                                                 bits_fs(W, (qb + 63) / 64) * min(p, lv),
                                                 bits_want_inner(p, qb, W, lv))
@@ -1638,7 +1643,7 @@ __global__ __launch_bounds__(kMaxWpg * kWave, 4) void bk_kernel(BkParams pr) {
         const uint32_t W = (p + 63) / 64;
         const bool bits = kIPX && q <= kMaxXRows;
         uint32_t qk = q;
-        const uint64_t keep = bits ? live_rows(pr, t.S, p, t.XL, q, qk) : 0ull;
+        const uint64_t keep = !bits ? 0ull : q > 2 * p ? live_rows(pr, t.S, p, t.XL, q, qk) : all_rows(q);
         if (!(bits ? place_bits(s, p, qk, W, rg.ptr, rg.bytes, hbm, pr.hbm_bytes, pr.stack_lds)
                    : place(s, p, q, W, kIPX, rg.ptr, rg.bytes, hbm, pr.hbm_bytes, pr.stack_lds))) {
           pool_put(pl, rg);
