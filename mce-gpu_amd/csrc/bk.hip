@@ -67,9 +67,11 @@ enum Acc : uint32_t {
 enum Stat : int { kStatNone = 0, kStatLb = 1, kStatDonor = 2, kStatBd = 3 };
 enum Bd : int { kBdOther = 0, kBdInduced, kBdQueue, kBdPivot, kBdSetOps, kBdMaximal, kBdBranch };
 
-// Idle-queue control words (head, tail) and the outstanding-work counter,
-// each on its own 64-byte line.
-constexpr uint32_t kCtlHead = 0, kCtlTail = 8, kCtlOutstanding = 16, kCtlWords = 32;
+// One idle queue per XCD (its head and tail each on a line of their own from
+// word kCtlQueues + 16q), so that donors and idle waves of different XCDs do
+// not contend for one line.
+constexpr uint32_t kQueues = 8;
+constexpr uint32_t kCtlOutstanding = 16, kCtlQueues = 32, kCtlWords = kCtlQueues + 16 * kQueues;
 // Mailbox words of the waves, each on its own 128-byte line.
 constexpr uint32_t kMboxStride = 16;
 constexpr unsigned long long kNoSlot = ~0ull;
@@ -158,6 +160,8 @@ struct WaveAcc {
   uint64_t sub_cliques = 0;
   int32_t budget = 0;
   uint64_t last_poll = 0;
+  uint32_t xq = 0;      // this wave's idle queue (its XCD)
+  uint32_t peek = 0;    // another queue sampled at the next poll
   uint64_t t_task = 0;  // realtime at the start of the current task
 };
 
@@ -624,12 +628,13 @@ __device__ __forceinline__ unsigned long long ld_relaxed(unsigned long long *p) 
 
 // Claims a queue position to publish into (lane 0 only). kNoSlot when full or
 // contended.
-__device__ __forceinline__ unsigned long long ring_claim_push(const BkParams &pr) {
-  unsigned long long *tail = pr.ctl + kCtlTail;
+__device__ __forceinline__ unsigned long long ring_claim_push(const BkParams &pr, uint32_t q) {
+  unsigned long long *tail = pr.ctl + kCtlQueues + 16 * q + 8;
+  unsigned long long *seq = pr.seq + (uint64_t)q * pr.ring_cap;
   unsigned long long pos = ld_relaxed(tail);
   for (int it = 0; it < 8; ++it) {
     const unsigned long long sq =
-        __hip_atomic_load(&pr.seq[pos % pr.ring_cap], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_load(&seq[pos % pr.ring_cap], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     const long long dif = (long long)(sq - pos);
     if (dif == 0) {
       if (__hip_atomic_compare_exchange_weak(tail, &pos, pos + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
@@ -646,12 +651,13 @@ __device__ __forceinline__ unsigned long long ring_claim_push(const BkParams &pr
 
 // Claims a published queue position to consume (lane 0 only). kNoSlot when
 // empty or contended.
-__device__ __forceinline__ unsigned long long ring_claim_pop(const BkParams &pr) {
-  unsigned long long *head = pr.ctl + kCtlHead;
+__device__ __forceinline__ unsigned long long ring_claim_pop(const BkParams &pr, uint32_t q) {
+  unsigned long long *head = pr.ctl + kCtlQueues + 16 * q;
+  unsigned long long *seq = pr.seq + (uint64_t)q * pr.ring_cap;
   unsigned long long pos = ld_relaxed(head);
   for (int it = 0; it < 8; ++it) {
     const unsigned long long sq =
-        __hip_atomic_load(&pr.seq[pos % pr.ring_cap], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_load(&seq[pos % pr.ring_cap], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     const long long dif = (long long)(sq - (pos + 1));
     if (dif == 0) {
       if (__hip_atomic_compare_exchange_weak(head, &pos, pos + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
@@ -666,26 +672,40 @@ __device__ __forceinline__ unsigned long long ring_claim_pop(const BkParams &pr)
   return kNoSlot;
 }
 
-// Idle waves waiting for work: the length of the idle queue (lane 0 samples).
-__device__ __forceinline__ int32_t donation_budget(const BkParams &pr) {
+__device__ __forceinline__ uint32_t queue_len(const BkParams &pr, uint32_t q) {
+  const unsigned long long h = ld_relaxed(pr.ctl + kCtlQueues + 16 * q),
+                           t = ld_relaxed(pr.ctl + kCtlQueues + 16 * q + 8);
+  return t > h ? (uint32_t)min(t - h, 0x7fffffffull) : 0u;
+}
+
+// Idle waves waiting for work: the length of this wave's queue plus one
+// other queue, visited in turn (lane 0 samples).
+__device__ __forceinline__ int32_t donation_budget(const BkParams &pr, WaveAcc &wa) {
   int32_t b = 0;
+  wa.peek = (wa.peek + 1) % kQueues;
   if (lane_id() == 0) {
-    const unsigned long long h = ld_relaxed(pr.ctl + kCtlHead), t = ld_relaxed(pr.ctl + kCtlTail);
-    b = (int32_t)(t > h ? t - h : 0);
+    b = (int32_t)queue_len(pr, wa.xq);
+    if (wa.peek != wa.xq) b += (int32_t)queue_len(pr, wa.peek);
   }
   return (int32_t)uniform((uint32_t)b);
 }
 
-// Takes an idle wave off the idle queue for a donation and counts the task as
-// outstanding; returns that wave's id (its slot receives the task) or kNone.
-__device__ __forceinline__ uint32_t claim_receiver(const BkParams &pr) {
+// Takes an idle wave off the idle queues (this XCD's first) for a donation
+// and counts the task as outstanding; returns that wave's id (its slot
+// receives the task) or kNone.
+__device__ __forceinline__ uint32_t claim_receiver(const BkParams &pr, uint32_t xq) {
   uint32_t r = kNone;
   if (lane_id() == 0) {
-    const unsigned long long pos = ring_claim_pop(pr);
-    if (pos != kNoSlot) {
-      r = __hip_atomic_load(&pr.idle_ids[pos % pr.ring_cap], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&pr.seq[pos % pr.ring_cap], pos + pr.ring_cap, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      atomicAdd(pr.ctl + kCtlOutstanding, 1ull);
+    for (uint32_t i = 0; i < kQueues && r == kNone; ++i) {
+      const uint32_t q = (xq + i) % kQueues;
+      if (i && queue_len(pr, q) == 0) continue;
+      const unsigned long long pos = ring_claim_pop(pr, q);
+      if (pos != kNoSlot) {
+        const uint64_t at = (uint64_t)q * pr.ring_cap + pos % pr.ring_cap;
+        r = __hip_atomic_load(&pr.idle_ids[at], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&pr.seq[at], pos + pr.ring_cap, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        atomicAdd(pr.ctl + kCtlOutstanding, 1ull);
+      }
     }
   }
   return uniform(r);
@@ -698,16 +718,17 @@ __device__ __forceinline__ void publish_task(const BkParams &pr, uint32_t r) {
     __hip_atomic_store(&pr.mbox[(uint64_t)r * kMboxStride], 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Puts this idle wave on the idle queue (its mailbox cleared first); false
-// when the queue's tail was too contended this time.
-__device__ __forceinline__ bool register_idle(const BkParams &pr, uint32_t gw) {
+// Puts this idle wave on its XCD's idle queue (its mailbox cleared first);
+// false when the queue's tail was too contended this time.
+__device__ __forceinline__ bool register_idle(const BkParams &pr, uint32_t gw, uint32_t xq) {
   unsigned long long pos = kNoSlot;
   if (lane_id() == 0) {
     __hip_atomic_store(&pr.mbox[(uint64_t)gw * kMboxStride], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    pos = ring_claim_push(pr);
+    pos = ring_claim_push(pr, xq);
     if (pos != kNoSlot) {
-      __hip_atomic_store(&pr.idle_ids[pos % pr.ring_cap], gw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&pr.seq[pos % pr.ring_cap], pos + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      const uint64_t at = (uint64_t)xq * pr.ring_cap + pos % pr.ring_cap;
+      __hip_atomic_store(&pr.idle_ids[at], gw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&pr.seq[at], pos + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
   return uniform64(pos) != kNoSlot;
@@ -762,7 +783,7 @@ __device__ __forceinline__ bool donate_shallowest(const BkParams &pr, Sub &s, ui
       nx += popc64(ballot(i < xej && xe_adj<kIPX>(s, s.XL[i], c, nb, nlen)));
     }
     if (nx > pr.slot_xcap || hbm_need(s.p, nx, kIPX) > pr.hbm_bytes) continue;
-    const uint32_t rcv = claim_receiver(pr);
+    const uint32_t rcv = claim_receiver(pr, wa.xq);
     if (rcv == kNone) return false;
     uint8_t *slot = pr.ring + (uint64_t)rcv * pr.slot_bytes;
     if (L == 0) {
@@ -847,7 +868,7 @@ __device__ __forceinline__ void bk_solve(const BkParams &pr, Sub &s, uint64_t cP
         wa.last_poll = wa.nodes;
         clk.to(kBdQueue);
         // only a task that has run for a while gives work away
-        wa.budget = __builtin_amdgcn_s_memrealtime() - wa.t_task >= pr.donate_age ? donation_budget(pr) : 0;
+        wa.budget = __builtin_amdgcn_s_memrealtime() - wa.t_task >= pr.donate_age ? donation_budget(pr, wa) : 0;
       }
       if (wa.budget > 0) {
         clk.to(kBdQueue);
@@ -1230,7 +1251,7 @@ __device__ __forceinline__ bool donate_bits(const BkParams &pr, Sub &s, uint32_t
     const uint64_t Pn = Pj & Ac, Xn = Xj & Ac;
     if (wave_sum(popc64(Pn)) < pr.donate_min_p) continue;
     const uint64_t En = filter_rows<kLX>(s, Ej, c);
-    const uint32_t rcv = claim_receiver(pr);
+    const uint32_t rcv = claim_receiver(pr, wa.xq);
     if (rcv == kNone) return false;
     uint8_t *slot = pr.ring + (uint64_t)rcv * pr.slot_bytes;
     if (L == 0) {
@@ -1308,7 +1329,7 @@ __device__ __forceinline__ void solve_bits_t(const BkParams &pr, Sub &s, uint64_
         wa.last_poll = wa.nodes;
         clk.to(kBdQueue);
         // only a task that has run for a while gives work away
-        wa.budget = __builtin_amdgcn_s_memrealtime() - wa.t_task >= pr.donate_age ? donation_budget(pr) : 0;
+        wa.budget = __builtin_amdgcn_s_memrealtime() - wa.t_task >= pr.donate_age ? donation_budget(pr, wa) : 0;
       }
       if (wa.budget > 0) {
         clk.to(kBdQueue);
@@ -1536,6 +1557,8 @@ __global__ __launch_bounds__(kMaxWpg * kWave, 4) void bk_kernel(BkParams pr) {
   const uint32_t gw = blockIdx.x * pr.wpg + wig;
   uint8_t *hbm = pr.hbm + (uint64_t)gw * pr.hbm_bytes;
   WaveAcc wa;
+  wa.xq = (hw_cu_id() >> 8) % kQueues;
+  wa.peek = wa.xq;
   Clock<kStat> clk;
   clk.start();
   Sub s;
@@ -1680,7 +1703,7 @@ __global__ __launch_bounds__(kMaxWpg * kWave, 4) void bk_kernel(BkParams pr) {
     const uint8_t *slot = pr.ring + (uint64_t)gw * pr.slot_bytes;
     while (true) {
       clk.to(kBdQueue);
-      if (!registered) registered = register_idle(pr, gw);
+      if (!registered) registered = register_idle(pr, gw, wa.xq);
       if (registered) {
         const unsigned long long got = uniform64(
             L == 0 ? __hip_atomic_load(&pr.mbox[(uint64_t)gw * kMboxStride], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)
@@ -1845,7 +1868,7 @@ __global__ void reset_kernel(unsigned long long *acc, uint32_t n, uint32_t *tick
 __global__ void ring_init(unsigned long long *seq, uint32_t cap, unsigned long long *ctl, unsigned long long outstanding,
                           unsigned long long *mbox) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) {
-    seq[i] = i;
+    for (uint32_t q = 0; q < kQueues; ++q) seq[(uint64_t)q * cap + i] = i;
     mbox[(uint64_t)i * kMboxStride] = 0;
   }
   if (blockIdx.x == 0 && threadIdx.x < kCtlWords) ctl[threadIdx.x] = threadIdx.x == kCtlOutstanding ? outstanding : 0;
@@ -2088,9 +2111,9 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
     pr.donate_check = std::max<uint32_t>(cfg.donate_check_nodes, 1);
     pr.donate_age = (uint64_t)cfg.donate_age_us * 100;
     if ((rc = dmalloc((void **)&pr.ctl, sizeof(unsigned long long) * kCtlWords))) return rc;
-    if ((rc = dmalloc((void **)&pr.seq, sizeof(unsigned long long) * pr.ring_cap))) return rc;
+    if ((rc = dmalloc((void **)&pr.seq, sizeof(unsigned long long) * kQueues * pr.ring_cap))) return rc;
     if ((rc = dmalloc((void **)&pr.ring, pr.slot_bytes * pr.ring_cap))) return rc;
-    if ((rc = dmalloc((void **)&pr.idle_ids, sizeof(uint32_t) * pr.ring_cap))) return rc;
+    if ((rc = dmalloc((void **)&pr.idle_ids, sizeof(uint32_t) * kQueues * pr.ring_cap))) return rc;
     if ((rc = dmalloc((void **)&pr.mbox, sizeof(unsigned long long) * kMboxStride * pr.ring_cap))) return rc;
     if (!cfg.capture) ring_init<<<64, 256, 0, s>>>(pr.seq, pr.ring_cap, pr.ctl, n_tickets, pr.mbox);
   }
