@@ -898,9 +898,10 @@ __device__ __forceinline__ void bk_solve(const BkParams &pr, Sub &s, uint64_t cP
     const uint32_t k0 = ctz64(bm);
     const uint32_t bit = ctz64(rdlane64(B, k0));
     const uint32_t c = k0 * 64 + bit;
-    if (L == k0) {
-      cP &= ~(1ull << bit);
-      cX |= 1ull << bit;
+    {
+      const uint64_t m = L == k0 ? (1ull << bit) : 0ull;
+      cP ^= m;
+      cX |= m;
     }
     const uint64_t Ac = act ? s.A[(uint64_t)c * W + L] : 0;
     const uint64_t Pn = cP & Ac, Xn = cX & Ac;
@@ -1352,9 +1353,12 @@ __device__ __forceinline__ void solve_bits_t(const BkParams &pr, Sub &s, uint64_
     const uint32_t k0 = ctz64(bm);
     const uint32_t bit = ctz64(rdlane64(B, k0));
     const uint32_t c = k0 * 64 + bit;
-    if (L == k0) {
-      cP &= ~(1ull << bit);
-      cX |= 1ull << bit;
+    {
+      // c leaves P for X: one lane-select of the bit, then XOR (the bit is
+      // set in P) and OR
+      const uint64_t m = L == k0 ? (1ull << bit) : 0ull;
+      cP ^= m;
+      cX |= m;
     }
     const uint64_t Ac = act ? ld64<kLA>(s.A, (uint64_t)c * W + L) : 0;
     const uint64_t Pn = cP & Ac, Xn = cX & Ac;
