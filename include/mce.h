@@ -117,6 +117,10 @@ typedef struct mce_opts {
                               roots cost more than they balance)            */
   int graph_launch;        /* 1 = run the enumeration as one launch captured
                               into a hipGraph (what mce_run_batch does)     */
+  int small_universe;      /* bitmap path: a node whose P, X and outside
+                              rows number at most this many has its whole
+                              subtree solved with one lane per member
+                              (1..64; 0 = 64; -1 = never)                   */
 
   int schedule;            /* enum mce_schedule                              */
   const uint32_t *order;   /* MCE_SCHED_GIVEN: first-level vertex order      */

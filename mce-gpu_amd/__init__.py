@@ -67,7 +67,7 @@ class Opts(ctypes.Structure):
         ("level", c_int), ("induced", c_int), ("worklist", c_int), ("task", c_int), ("device", c_int),
         ("waves_per_group", c_int), ("groups_per_cu", c_int), ("lds_arena_bytes", c_int),
         ("hbm_arena_bytes", c_int), ("stack_lds_levels", c_int), ("donate_min_p", c_int),
-        ("donate_check_nodes", c_int), ("donate_age_us", c_int), ("split_heavy", c_int), ("graph_launch", c_int), ("schedule", c_int), ("order", POINTER(c_uint32)),
+        ("donate_check_nodes", c_int), ("donate_age_us", c_int), ("split_heavy", c_int), ("graph_launch", c_int), ("small_universe", c_int), ("schedule", c_int), ("order", POINTER(c_uint32)),
         ("order_len", c_uint64), ("model_path", c_char_p), ("model", POINTER(Mlp)), ("sub_nodes", POINTER(c_uint64)),
         ("sub_cliques", POINTER(c_uint64)),
     ]

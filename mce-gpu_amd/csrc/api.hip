@@ -88,6 +88,7 @@ int check_opts(const mce_opts *o) {
     return set_error(MCE_EINVAL, "lds_arena_bytes must be a multiple of 8 and >= 1024");
   if (o->split_heavy < -1) return set_error(MCE_EINVAL, "split_heavy must be -1, 0 or a node count");
   if (o->graph_launch != 0 && o->graph_launch != 1) return set_error(MCE_EINVAL, "graph_launch must be 0 or 1");
+  if (o->small_universe < -1 || o->small_universe > 64) return set_error(MCE_EINVAL, "small_universe must be -1..64");
   if (o->schedule < MCE_SCHED_PLAIN || o->schedule > MCE_SCHED_GIVEN) return set_error(MCE_EINVAL, "unknown schedule");
   if (o->schedule == MCE_SCHED_GIVEN && !o->order) return set_error(MCE_EINVAL, "given schedule without an order");
   return MCE_OK;
@@ -413,6 +414,7 @@ static int count_impl(mce_graph *h, const mce_opts *opts, mce_result *res, bool 
   cfg.donate_check_nodes = opts->donate_check_nodes ? (uint32_t)opts->donate_check_nodes : 64;
   cfg.donate_age_us = opts->donate_age_us ? (uint32_t)opts->donate_age_us : 1000;
   cfg.capture = capture || opts->graph_launch;
+  cfg.small_max = opts->small_universe < 0 ? 0u : opts->small_universe ? (uint32_t)opts->small_universe : 64u;
   if (heavy_tail && !opts->donate_age_us) {
     // thresholds set from the cost model: with a predicted heavy tail only
     // long-running tasks donate (a donation copies the donor's bit matrix;

@@ -129,6 +129,7 @@ struct BkParams {
   uint32_t donate_min_p;
   uint32_t donate_check;
   uint64_t donate_age;  // 100 MHz realtime ticks a task runs before it may donate
+  uint32_t small_max;   // nodes with P + X + outside rows <= this go to the small-universe solver
 };
 
 struct Sub {
@@ -1310,6 +1311,166 @@ __device__ __forceinline__ bool donate_bits(const BkParams &pr, Sub &s, uint32_t
   return false;
 }
 
+// Member codes of the small-universe solver: bits 0..12 the position (S
+// index, or outside row with kMemOut), kMemP for a member of P.
+constexpr uint32_t kMemOut = 1u << 14, kMemP = 1u << 15;
+// Largest P + X + outside rows the small-universe solver takes (one lane per
+// member).
+constexpr uint32_t kSmallMax = 64;
+
+// Solves the whole subtree under one node whose P, X and outside rows E number
+// at most 64 in total. The members are renumbered in the order the bitmap
+// solver breaks ties in (S indices ascending, then outside rows ascending):
+// lane i holds member i's adjacency row over the members and, for an S
+// member, its hash term. P and X (which takes in the outside rows) become
+// wave-uniform 64-bit masks, a child is one lane read and two scalar ANDs,
+// the pivot one AND + popcount per lane and one reduction, and the frames sit
+// in lane d of seven registers. Pivot (largest |P & N(u)|, ties to the lowest
+// member) and child order (lowest member first) are the bitmap solver's, so
+// the same nodes are visited. rbase = |R| at the node; mix = R's hash sum.
+template <int kStat, bool kLA, bool kLX>
+__device__ __forceinline__ void solve_small(const BkParams &pr, const Sub &s, uint64_t Pw, uint64_t Xw, uint64_t Ew,
+                                            uint64_t mix, uint32_t rbase, unsigned long long *hist_lds, WaveAcc &wa,
+                                            Clock<kStat> &clk) {
+  const uint32_t L = lane_id(), W = s.W;
+  clk.to(kBdInduced);
+  // lane i <- code of member i: each nonzero word pushes its set bits to
+  // consecutive lanes (lanes without a bit push to a lane outside the range)
+  const uint64_t Cw = Pw | Xw;
+  uint32_t mem = 0, base = 0;
+  for (uint64_t cm = ballot(Cw != 0); cm; cm &= cm - 1) {
+    const uint32_t k = ctz64(cm);
+    const uint64_t w = rdlane64(Cw, k), pw = rdlane64(Pw, k);
+    const uint32_t cnt = popc64(w);
+    const bool in = (w >> L) & 1ull;
+    const uint32_t dst = in ? base + mbcnt(w) : (base + cnt) & 63u;
+    const uint32_t code = (k * 64 + L) | (((pw >> L) & 1ull) ? kMemP : 0u);
+    const uint32_t got = (uint32_t)__builtin_amdgcn_ds_permute((int)(dst * 4), (int)code);
+    if (L - base < cnt) mem = got;
+    base += cnt;
+  }
+  const uint32_t nS = base;
+  for (uint64_t em = ballot(Ew != 0); em; em &= em - 1) {
+    const uint32_t k = ctz64(em);
+    const uint64_t w = rdlane64(Ew, k);
+    const uint32_t cnt = popc64(w);
+    const bool in = (w >> L) & 1ull;
+    const uint32_t dst = in ? base + mbcnt(w) : (base + cnt) & 63u;
+    const uint32_t got = (uint32_t)__builtin_amdgcn_ds_permute((int)(dst * 4), (int)((k * 64 + L) | kMemOut));
+    if (L - base < cnt) mem = got;
+    base += cnt;
+  }
+  const uint32_t nm = base;
+  const bool valid = L < nm, out = (mem & kMemOut) != 0;
+  const uint32_t g = mem & 8191u;
+  const uint64_t lP = ballot(valid && (mem & kMemP)), lX = ballot(valid && !(mem & kMemP));
+  // rows over the S members: bit j of word k's members, in member order
+  uint32_t mlo = 0, mhi = 0, j = 0;
+  for (uint64_t cm = ballot(Cw != 0); cm; cm &= cm - 1) {
+    const uint32_t k = ctz64(cm);
+    const uint64_t w = rdlane64(Cw, k);
+    uint64_t rw = 0;
+    if (valid) rw = out ? ld64<kLX>(s.XR, (uint64_t)g * W + k) : ld64<kLA>(s.A, (uint64_t)g * W + k);
+    for (uint32_t h = 0; h < 2; ++h) {
+      const uint32_t r = (uint32_t)(rw >> (32 * h));
+      for (uint32_t wh = (uint32_t)(w >> (32 * h)); wh; wh &= wh - 1) {
+        const uint32_t t = (r >> __builtin_ctz(wh)) & 1u;
+        if (j < 32) mlo |= t << j;
+        else mhi |= t << (j - 32);
+        ++j;
+      }
+    }
+  }
+  uint64_t M = ((uint64_t)mhi << 32) | mlo;
+  // S members' bits for the outside rows: the transpose of the outside rows'
+  // S bits
+  for (uint32_t e = nS; e < nm; ++e) {
+    const uint64_t re = rdlane64(M, e);
+    M |= (L < nS ? (re >> L) & 1ull : 0ull) << e;
+  }
+  const uint64_t hs = valid && !out ? ld64<kLA>(s.H, g) : 0ull;
+
+  // pivot over P | X (the lowest member among the best scores): false when
+  // it covers P (no child)
+  auto pick = [&](uint64_t P, uint64_t X, uint32_t &pv) -> bool {
+    const uint32_t key = ((P | X) >> L) & 1ull ? popc64(M & P) + 1u : 0u;
+    const uint32_t mx = wave_max(key);
+    pv = ctz64(ballot(key == mx));
+    return mx != 0 && mx - 1u != popc64(P);
+  };
+  clk.to(kBdPivot);
+  uint64_t P = lP, X = lX;
+  uint32_t piv;
+  if (!pick(P, X, piv)) return;
+  uint64_t Np = rdlane64(M, piv);
+  uint32_t d = 0;
+  // frames: lane d of each register holds level d
+  uint32_t fPl = 0, fPh = 0, fXl = 0, fXh = 0, fV = 0, fMl = 0, fMh = 0;
+  // uniform tallies, added to the wave's totals at the end
+  uint64_t nodes = 0, found = 0, hsum = 0;
+  uint32_t top = 0;
+  while (true) {
+    clk.to(kBdSetOps);
+    const uint64_t B = P & ~Np;
+    if (B == 0) {
+      if (d == 0) break;
+      clk.to(kBdBranch);
+      --d;
+      P = ((uint64_t)rdlane(fPh, d) << 32) | rdlane(fPl, d);
+      X = ((uint64_t)rdlane(fXh, d) << 32) | rdlane(fXl, d);
+      mix = ((uint64_t)rdlane(fMh, d) << 32) | rdlane(fMl, d);
+      piv = rdlane(fV, d);
+      Np = rdlane64(M, piv);
+      continue;
+    }
+    const uint64_t cb = B & (~B + 1);
+    P ^= cb;
+    X |= cb;
+    const uint32_t c = ctz64(cb);
+    const uint64_t Mc = rdlane64(M, c);
+    const uint64_t Pn = P & Mc, Xn = X & Mc;
+    ++nodes;
+    const uint64_t cmix = mix + rdlane64(hs, c);
+    if (Pn == 0) {
+      if (Xn == 0) {
+        clk.to(kBdMaximal);
+        const uint32_t size = rbase + d + 1;
+        ++found;
+        hsum += clique_hash(cmix, size);
+        top = max(top, size);
+        if (L == 0) {
+          if (size < pr.hist_lds) atomicAdd(&hist_lds[size], 1ull);
+          else atomicAdd(&pr.hist[min(size, pr.hist_len - 1)], 1ull);
+        }
+      }
+      continue;
+    }
+    clk.to(kBdPivot);
+    uint32_t npiv;
+    if (!pick(Pn, Xn, npiv)) continue;
+    clk.to(kBdBranch);
+    const bool here = L == d;
+    fPl = here ? (uint32_t)P : fPl;
+    fPh = here ? (uint32_t)(P >> 32) : fPh;
+    fXl = here ? (uint32_t)X : fXl;
+    fXh = here ? (uint32_t)(X >> 32) : fXh;
+    fMl = here ? (uint32_t)mix : fMl;
+    fMh = here ? (uint32_t)(mix >> 32) : fMh;
+    fV = here ? piv : fV;
+    ++d;
+    P = Pn;
+    X = Xn;
+    mix = cmix;
+    piv = npiv;
+    Np = rdlane64(M, piv);
+  }
+  wa.nodes += nodes;
+  wa.count += found;
+  wa.sub_cliques += found;
+  wa.hash += hsum;
+  wa.max_size = max(wa.max_size, top);
+}
+
 // Runs BK from (P, X, E) on an arena that build_bits (or a received task)
 // has filled; kLA / kLX: S and A / the outside rows are in LDS.
 template <bool kWL, int kStat, bool kLA, bool kLX>
@@ -1317,6 +1478,10 @@ __device__ __forceinline__ void solve_bits_t(const BkParams &pr, Sub &s, uint64_
                                              unsigned long long *hist_lds, WaveAcc &wa, Clock<kStat> &clk) {
   const uint32_t L = lane_id(), W = s.W, WX = s.WX;
   const bool act = L < W;
+  if (wave_sum(popc64(cP) + popc64(cX) + popc64(cE)) <= pr.small_max) {
+    solve_small<kStat, kLA, kLX>(pr, s, cP, cX, cE, s.rmix, s.rsize, hist_lds, wa, clk);
+    return;
+  }
   uint32_t piv, best;
   clk.to(kBdPivot);
   pick_pivot_bits<kLA, kLX>(s, cP, cX, cE, piv, best);
@@ -1433,6 +1598,10 @@ __device__ __forceinline__ void solve_bits_t(const BkParams &pr, Sub &s, uint64_
         if (ballot((Xc | Ec) != 0) == 0) report(pr, hist_lds, wa, s.rsize + d + 2, cmix + (piv_a ? hb : ha));
         continue;
       }
+    }
+    if (wave_sum(popc64(Pn) + popc64(Xn) + popc64(En)) <= pr.small_max) {
+      solve_small<kStat, kLA, kLX>(pr, s, Pn, Xn, En, cmix, s.rsize + d + 1, hist_lds, wa, clk);
+      continue;
     }
     uint32_t npiv, nbest;
     clk.to(kBdPivot);
@@ -2077,6 +2246,7 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   pr.pool_bytes = (uint32_t)pool;
   pr.stack_lds = cfg.stack_lds_levels;
   pr.want_levels = cfg.stack_lds_levels ? cfg.stack_lds_levels : 24;
+  pr.small_max = std::min<uint32_t>(cfg.small_max, kSmallMax);
   pr.hbm = hbm;
   pr.hbm_bytes = arena;
   pr.acc = acc;

@@ -48,6 +48,7 @@ void usage() {
           "\n    --donate-min-p N      donate a child branch only if |P| >= N (default 10)"
           "\n    --donate-check N      tree nodes between idle-worker polls"
           "\n    --donate-age US       a task donates only after running this long"
+          "\n    --small-universe N    nodes with |P|+|X| <= N (1..64) run one lane per member; -1 = never"
           "\n    --schedule plain|learned   first-level order (default plain)"
           "\n    --model PATH          cost model for --schedule learned"
           "\n"
@@ -98,6 +99,7 @@ int main(int argc, char **argv) {
                                      {"schedule", required_argument, 0, 7},
                                      {"model", required_argument, 0, 8},
                                      {"donate-age", required_argument, 0, 9},
+                                     {"small-universe", required_argument, 0, 10},
                                      {0, 0, 0, 0}};
   int opt;
   while ((opt = getopt_long(argc, argv, "g:r:d:m:p:i:w:h", longopts, nullptr)) >= 0) {
@@ -156,6 +158,7 @@ int main(int argc, char **argv) {
         break;
       case 8: model = optarg; break;
       case 9: opts.donate_age_us = atoi(optarg); break;
+      case 10: opts.small_universe = atoi(optarg); break;
       default:
         fprintf(stderr, "\nUnrecognized option!\n");
         usage();

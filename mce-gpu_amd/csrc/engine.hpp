@@ -55,6 +55,7 @@ struct BkLaunch {
   uint32_t donate_min_p = 10;
   uint32_t donate_check_nodes = 64;
   uint32_t donate_age_us = 1000;
+  uint32_t small_max = 64;  // P + X + outside rows at which a node goes to the small-universe solver (0 = never)
   bool capture = false;  // run the enumeration launch as a captured hipGraph
 };
 

@@ -93,7 +93,7 @@ def test_cli_auto_variant_and_engine_options():
     # -w auto and the engine's long options parse; the run then stops at the
     # missing graph file, not at the options
     r = cli("-m", "mce", "-g", "/nonexistent.bel", "-d", "0", "-p", "l1", "-i", "px", "-w", "auto",
-            "--donate-age", "5000", "--waves-per-group", "8", "--schedule", "learned")
+            "--donate-age", "5000", "--waves-per-group", "8", "--schedule", "learned", "--small-universe", "32")
     assert r.returncode == 0
     assert "Unrecognized" not in r.stderr and "Must specify" not in r.stderr
     assert "Worker List Scheme = Auto (cost model)" in r.stdout
@@ -103,6 +103,7 @@ def test_cli_auto_variant_and_engine_options():
 def test_cli_help_and_convert(tmp_path):
     r = cli("-h")
     assert r.returncode == 0 and "Usage:" in r.stderr and "--lds-arena" in r.stderr and "--donate-age" in r.stderr
+    assert "--small-universe" in r.stderr
     txt = tmp_path / "e.txt"
     txt.write_text("0 1\n1 2\n2 0\n2 3\n")
     out = tmp_path / "e.bel"

@@ -75,6 +75,19 @@ def test_multiword_bitmaps(mce, oracle):
         run_all(mce, oracle, rand_graph(mce, n, p, 1), f"dense-{n}")
 
 
+@pytest.mark.parametrize("small", [-1, 8, 33])
+def test_small_universe_thresholds(mce, oracle, small):
+    # nodes with |P| + |X| + outside rows <= small run one lane per member;
+    # -1 keeps every node on the word-per-lane solver, 8 and 33 switch deep
+    # in the tree (and 33 at the root of the sparser graph)
+    for n, p, seed in [(200, 0.5, 1), (120, 0.3, 2)]:
+        print(f"case small={small} n={n} p={p}", flush=True)
+        run_all(mce, oracle, rand_graph(mce, n, p, seed), f"small{small}-{n}", small_universe=small)
+    print(f"case small={small} hbm arena", flush=True)
+    run_all(mce, oracle, rand_graph(mce, 220, 0.55, 7), f"small{small}-hbm", small_universe=small,
+            lds_arena_bytes=1024)
+
+
 def test_hbm_arena_and_frame_spill(mce, oracle):
     # a tiny LDS arena forces the HBM arena and HBM stack frames
     g = rand_graph(mce, 220, 0.55, 7)
