@@ -1878,8 +1878,11 @@ __global__ __launch_bounds__(kMaxWpg * kWave, 4) void bk_kernel(BkParams pr) {
       clk.to(kBdQueue);
       if (!registered) registered = register_idle(pr, gw, wa.xq);
       if (registered) {
+        // relaxed poll (an acquire at agent scope would invalidate the CU's
+        // vector L1 on every poll, under the waves still at work); the
+        // acquire fence follows only once a task is in
         const unsigned long long got = uniform64(
-            L == 0 ? __hip_atomic_load(&pr.mbox[(uint64_t)gw * kMboxStride], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)
+            L == 0 ? __hip_atomic_load(&pr.mbox[(uint64_t)gw * kMboxStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                    : 0ull);
         if (got) {
           registered = false;
