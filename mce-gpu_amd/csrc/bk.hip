@@ -1406,9 +1406,9 @@ __device__ __forceinline__ void solve_small(const BkParams &pr, const Sub &s, ui
   uint32_t d = 0;
   // frames: lane d of each register holds level d
   uint32_t fPl = 0, fPh = 0, fXl = 0, fXh = 0, fV = 0, fMl = 0, fMh = 0;
-  // uniform tallies, added to the wave's totals at the end
-  uint64_t nodes = 0, found = 0, hsum = 0;
-  uint32_t top = 0;
+  // tallies of this subtree, added to the wave's totals at the end
+  WaveAcc t{};
+  uint64_t &nodes = t.nodes;
   while (true) {
     clk.to(kBdSetOps);
     const uint64_t B = P & ~Np;
@@ -1432,17 +1432,44 @@ __device__ __forceinline__ void solve_small(const BkParams &pr, const Sub &s, ui
     ++nodes;
     const uint64_t cmix = mix + rdlane64(hs, c);
     if (Pn == 0) {
-      if (Xn == 0) {
-        clk.to(kBdMaximal);
-        const uint32_t size = rbase + d + 1;
-        ++found;
-        hsum += clique_hash(cmix, size);
-        top = max(top, size);
-        if (L == 0) {
-          if (size < pr.hist_lds) atomicAdd(&hist_lds[size], 1ull);
-          else atomicAdd(&pr.hist[min(size, pr.hist_len - 1)], 1ull);
+      clk.to(kBdMaximal);
+      if (Xn == 0) report(pr, hist_lds, t, rbase + d + 1, cmix);
+      continue;
+    }
+    const uint32_t np = popc64(Pn);
+    if (np <= 2) {
+      // |P| <= 2 decided without a pivot scan, node for node as the scan
+      // would go (see the word-per-lane solver)
+      clk.to(kBdMaximal);
+      const uint32_t a = ctz64(Pn);
+      const uint64_t Ma = rdlane64(M, a), ha = rdlane64(hs, a);
+      if (np == 1) {
+        if ((Xn & Ma) == 0) {
+          ++nodes;
+          report(pr, hist_lds, t, rbase + d + 2, cmix + ha);
         }
+        continue;
       }
+      const uint32_t b = ctz64(Pn & (Pn - 1));
+      const uint64_t Mb = rdlane64(M, b), hb = rdlane64(hs, b);
+      if (Xn & Ma & Mb) continue;  // an X member covers P
+      if ((Ma >> b) & 1ull) {
+        nodes += 2;
+        report(pr, hist_lds, t, rbase + d + 3, cmix + ha + hb);
+        continue;
+      }
+      const uint64_t xs = Xn & (Ma | Mb);
+      if (xs == 0) {
+        nodes += 2;
+        report(pr, hist_lds, t, rbase + d + 2, cmix + ha);
+        report(pr, hist_lds, t, rbase + d + 2, cmix + hb);
+        continue;
+      }
+      // the pivot is the lowest X member adjacent to one of a, b; the child
+      // is the other one
+      const bool piv_a = (Ma >> ctz64(xs)) & 1ull;
+      ++nodes;
+      if ((Xn & (piv_a ? Mb : Ma)) == 0) report(pr, hist_lds, t, rbase + d + 2, cmix + (piv_a ? hb : ha));
       continue;
     }
     clk.to(kBdPivot);
@@ -1464,11 +1491,11 @@ __device__ __forceinline__ void solve_small(const BkParams &pr, const Sub &s, ui
     piv = npiv;
     Np = rdlane64(M, piv);
   }
-  wa.nodes += nodes;
-  wa.count += found;
-  wa.sub_cliques += found;
-  wa.hash += hsum;
-  wa.max_size = max(wa.max_size, top);
+  wa.nodes += t.nodes;
+  wa.count += t.count;
+  wa.sub_cliques += t.count;
+  wa.hash += t.hash;
+  wa.max_size = max(wa.max_size, t.max_size);
 }
 
 // Runs BK from (P, X, E) on an arena that build_bits (or a received task)
