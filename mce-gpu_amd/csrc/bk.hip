@@ -130,6 +130,7 @@ struct BkParams {
   uint32_t donate_check;
   uint64_t donate_age;  // 100 MHz realtime ticks a task runs before it may donate
   uint32_t small_max;   // nodes with P + X + outside rows <= this go to the small-universe solver
+  uint32_t small_pmax;  // ... and, with the worker list, |P| <= this
 };
 
 struct Sub {
@@ -1222,6 +1223,51 @@ __device__ __forceinline__ uint64_t pivot_word(const Sub &s, uint32_t piv) {
   return piv < s.p ? ld64<kLA>(s.A, (uint64_t)piv * s.W + L) : ld64<kLX>(s.XR, (uint64_t)(piv - s.p) * s.W + L);
 }
 
+// Hands the node (P, X, E) with |R| = rsize and R's hash sum rmix to an idle
+// wave (bitmap path): the donor's S and A, the node's P and X words, and the
+// outside rows live in E packed in row order (which keeps the pivot's tie
+// rule), renumbered 0..q'-1. False when no wave is idle.
+template <bool kLX>
+__device__ __forceinline__ bool send_bits(const BkParams &pr, const Sub &s, uint32_t rsize, uint64_t rmix,
+                                          uint64_t Pn, uint64_t Xn, uint64_t En, WaveAcc &wa) {
+  const uint32_t L = lane_id(), W = s.W;
+  const uint32_t rcv = claim_receiver(pr, wa.xq);
+  if (rcv == kNone) return false;
+  uint8_t *slot = pr.ring + (uint64_t)rcv * pr.slot_bytes;
+  if (L == 0) {
+    uint32_t *h = (uint32_t *)slot;
+    h[0] = s.root;
+    h[1] = rsize;
+    h[2] = s.p;
+    h[3] = W;
+    h[4] = 0;  // outside rows: set once they are packed
+    h[5] = kTaskBits;
+    *(uint64_t *)(slot + 24) = rmix;
+  }
+  uint64_t *sp = (uint64_t *)(slot + pr.slot_oP);
+  if (L < W) {
+    sp[L] = Pn;
+    sp[pr.slot_wmax + L] = Xn;
+  }
+  wave_copy((uint32_t *)(slot + pr.slot_oS), s.S, s.p);
+  copy64((uint64_t *)(slot + pr.slot_oA), s.A, (uint64_t)s.p * W);
+  uint64_t *sXR = (uint64_t *)(slot + pr.slot_oXR);
+  uint32_t qn = 0;
+  for (uint64_t em = ballot(En != 0); em; em &= em - 1) {
+    const uint32_t k = ctz64(em);
+    const uint64_t ek = rdlane64(En, k);
+    if ((ek >> L) & 1ull) {
+      const uint64_t src = (uint64_t)(k * 64 + L) * W, dst = (uint64_t)(qn + mbcnt(ek)) * W;
+      for (uint32_t w = 0; w < W; ++w) sXR[dst + w] = ld64<kLX>(s.XR, src + w);
+    }
+    qn += popc64(ek);
+  }
+  if (L < (qn + 63) / 64) sp[2 * pr.slot_wmax + L] = full_word(qn);
+  if (L == 0) ((uint32_t *)slot)[4] = qn;
+  publish_task(pr, rcv);
+  return true;
+}
+
 // Publishes the next child of the shallowest frame with a large enough child
 // (bitmap path): the donor's S, A and outside rows plus the child's P, X, E.
 template <bool kLA, bool kLX>
@@ -1253,42 +1299,7 @@ __device__ __forceinline__ bool donate_bits(const BkParams &pr, Sub &s, uint32_t
     const uint64_t Pn = Pj & Ac, Xn = Xj & Ac;
     if (wave_sum(popc64(Pn)) < pr.donate_min_p) continue;
     const uint64_t En = filter_rows<kLX>(s, Ej, c);
-    const uint32_t rcv = claim_receiver(pr, wa.xq);
-    if (rcv == kNone) return false;
-    uint8_t *slot = pr.ring + (uint64_t)rcv * pr.slot_bytes;
-    if (L == 0) {
-      uint32_t *h = (uint32_t *)slot;
-      h[0] = s.root;
-      h[1] = s.rsize + j + 1;
-      h[2] = s.p;
-      h[3] = W;
-      h[4] = 0;  // outside rows: set once they are packed
-      h[5] = kTaskBits;
-      *(uint64_t *)(slot + 24) = mixj + s.H[c];
-    }
-    uint64_t *sp = (uint64_t *)(slot + pr.slot_oP);
-    if (act) {
-      sp[L] = Pn;
-      sp[pr.slot_wmax + L] = Xn;
-    }
-    wave_copy((uint32_t *)(slot + pr.slot_oS), s.S, s.p);
-    copy64((uint64_t *)(slot + pr.slot_oA), s.A, (uint64_t)s.p * W);
-    // only the outside rows live in the child travel, in row order (which
-    // keeps the pivot's tie rule), renumbered 0..q'-1
-    uint64_t *sXR = (uint64_t *)(slot + pr.slot_oXR);
-    uint32_t qn = 0;
-    for (uint64_t em = ballot(En != 0); em; em &= em - 1) {
-      const uint32_t k = ctz64(em);
-      const uint64_t ek = rdlane64(En, k);
-      if ((ek >> L) & 1ull) {
-        const uint64_t src = (uint64_t)(k * 64 + L) * W, dst = (uint64_t)(qn + mbcnt(ek)) * W;
-        for (uint32_t w = 0; w < W; ++w) sXR[dst + w] = ld64<kLX>(s.XR, src + w);
-      }
-      qn += popc64(ek);
-    }
-    if (L < (qn + 63) / 64) sp[2 * pr.slot_wmax + L] = full_word(qn);
-    if (L == 0) ((uint32_t *)slot)[4] = qn;
-    publish_task(pr, rcv);
+    if (!send_bits<kLX>(pr, s, s.rsize + j + 1, mixj + s.H[c], Pn, Xn, En, wa)) return false;
     const uint64_t m = 1ull << bit;
     if (j == d) {
       if (L == k0) {
@@ -1328,7 +1339,20 @@ constexpr uint32_t kSmallMax = 64;
 // in lane d of seven registers. Pivot (largest |P & N(u)|, ties to the lowest
 // member) and child order (lowest member first) are the bitmap solver's, so
 // the same nodes are visited. rbase = |R| at the node; mix = R's hash sum.
-template <int kStat, bool kLA, bool kLX>
+// Does a node go to the small-universe solver? P + X + outside rows within
+// the threshold; with the worker list also |P| <= kSmallMaxPWl, since a
+// small-universe subtree is never split between waves and its size is
+// bounded by |P| (at most 3^(|P|/3) maximal cliques). 32 measured best of
+// 16..40 on R-MAT 14, Chung-Lu 2^19 and a 32-graph SBM batch.
+constexpr uint32_t kSmallMaxPWl = 32;
+template <bool kWL>
+__device__ __forceinline__ bool small_ok(const BkParams &pr, uint64_t P, uint64_t X, uint64_t E) {
+  const uint32_t np = popc64(P);
+  if (kWL && wave_sum(np) > pr.small_pmax) return false;
+  return wave_sum(np + popc64(X) + popc64(E)) <= pr.small_max;
+}
+
+template <bool kWL, int kStat, bool kLA, bool kLX>
 __device__ __forceinline__ void solve_small(const BkParams &pr, const Sub &s, uint64_t Pw, uint64_t Xw, uint64_t Ew,
                                             uint64_t mix, uint32_t rbase, unsigned long long *hist_lds, WaveAcc &wa,
                                             Clock<kStat> &clk) {
@@ -1505,8 +1529,8 @@ __device__ __forceinline__ void solve_bits_t(const BkParams &pr, Sub &s, uint64_
                                              unsigned long long *hist_lds, WaveAcc &wa, Clock<kStat> &clk) {
   const uint32_t L = lane_id(), W = s.W, WX = s.WX;
   const bool act = L < W;
-  if (wave_sum(popc64(cP) + popc64(cX) + popc64(cE)) <= pr.small_max) {
-    solve_small<kStat, kLA, kLX>(pr, s, cP, cX, cE, s.rmix, s.rsize, hist_lds, wa, clk);
+  if (small_ok<kWL>(pr, cP, cX, cE)) {
+    solve_small<kWL, kStat, kLA, kLX>(pr, s, cP, cX, cE, s.rmix, s.rsize, hist_lds, wa, clk);
     return;
   }
   uint32_t piv, best;
@@ -1626,8 +1650,8 @@ __device__ __forceinline__ void solve_bits_t(const BkParams &pr, Sub &s, uint64_
         continue;
       }
     }
-    if (wave_sum(popc64(Pn) + popc64(Xn) + popc64(En)) <= pr.small_max) {
-      solve_small<kStat, kLA, kLX>(pr, s, Pn, Xn, En, cmix, s.rsize + d + 1, hist_lds, wa, clk);
+    if (small_ok<kWL>(pr, Pn, Xn, En)) {
+      solve_small<kWL, kStat, kLA, kLX>(pr, s, Pn, Xn, En, cmix, s.rsize + d + 1, hist_lds, wa, clk);
       continue;
     }
     uint32_t npiv, nbest;
@@ -2277,6 +2301,7 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   pr.stack_lds = cfg.stack_lds_levels;
   pr.want_levels = cfg.stack_lds_levels ? cfg.stack_lds_levels : 24;
   pr.small_max = std::min<uint32_t>(cfg.small_max, kSmallMax);
+  pr.small_pmax = kSmallMaxPWl;
   pr.hbm = hbm;
   pr.hbm_bytes = arena;
   pr.acc = acc;
