@@ -202,6 +202,8 @@ def main():
             # block settings (and, left to it, the variant) timed on a sample
             # of this graph; untimed set-up like the warmup
             t = time.perf_counter()
+            if not args.warmup:
+                dg.order()  # the tuner runs on the ordered graph
             best, trials = dg.autotune(sample_fraction=args.tune_sample, level=args.level, induced=args.induced,
                                        worklist=args.worklist)
             tuned = dict(best, trials=len(trials), seconds=time.perf_counter() - t)
