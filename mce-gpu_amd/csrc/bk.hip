@@ -1339,6 +1339,45 @@ constexpr uint32_t kSmallMax = 64;
 // in lane d of seven registers. Pivot (largest |P & N(u)|, ties to the lowest
 // member) and child order (lowest member first) are the bitmap solver's, so
 // the same nodes are visited. rbase = |R| at the node; mix = R's hash sum.
+// Maximal cliques found by the small-universe solver, buffered one per lane
+// (R's hash sum and size) so that the 64-bit hash mixing and the histogram
+// updates run on the vector unit 64 cliques at a time; hacc keeps each lane's
+// running sum of hash terms.
+struct CliqueBuf {
+  uint64_t mix = 0, hacc = 0;
+  uint32_t size = 0, n = 0;
+};
+
+__device__ __forceinline__ void flush_cliques(const BkParams &pr, unsigned long long *hist_lds, CliqueBuf &cb) {
+  if (lane_id() < cb.n) {
+    cb.hacc += clique_hash(cb.mix, cb.size);
+    if (cb.size < pr.hist_lds) atomicAdd(&hist_lds[cb.size], 1ull);
+    else atomicAdd(&pr.hist[min(cb.size, pr.hist_len - 1)], 1ull);
+  }
+  cb.n = 0;
+}
+
+// (the caller flushes before the buffer can overflow)
+__device__ __forceinline__ void keep_clique(const BkParams &pr, unsigned long long *hist_lds, WaveAcc &t,
+                                            CliqueBuf &cb, uint32_t size, uint64_t mix) {
+  if (lane_id() == cb.n) {
+    cb.mix = mix;
+    cb.size = size;
+  }
+  t.count++;
+  t.max_size = max(t.max_size, size);
+  cb.n++;
+}
+
+// Sum of a 64-bit value over the wave (uniform result).
+__device__ __forceinline__ uint64_t wave_add64(uint64_t v) {
+  for (int off = 1; off < (int)kWave; off <<= 1) {
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, off), hi = (uint32_t)__shfl_xor((int)(v >> 32), off);
+    v += ((uint64_t)hi << 32) | lo;
+  }
+  return uniform64(v);
+}
+
 // Does a node go to the small-universe solver? P + X + outside rows within
 // the threshold; with the worker list also |P| <= kSmallMaxPWl, since a
 // small-universe subtree is never split between waves and its size is
@@ -1430,10 +1469,14 @@ __device__ __forceinline__ void solve_small(const BkParams &pr, const Sub &s, ui
   uint32_t d = 0;
   // frames: lane d of each register holds level d
   uint32_t fPl = 0, fPh = 0, fXl = 0, fXh = 0, fV = 0, fMl = 0, fMh = 0;
-  // tallies of this subtree, added to the wave's totals at the end
+  // tallies of this subtree, added to the wave's totals at the end; the
+  // cliques' hash terms and histogram bins are taken 64 at a time
   WaveAcc t{};
   uint64_t &nodes = t.nodes;
+  CliqueBuf kb;
   while (true) {
+    // an iteration keeps at most two cliques
+    if (kb.n > kWave - 2) flush_cliques(pr, hist_lds, kb);
     clk.to(kBdSetOps);
     const uint64_t B = P & ~Np;
     if (B == 0) {
@@ -1457,7 +1500,7 @@ __device__ __forceinline__ void solve_small(const BkParams &pr, const Sub &s, ui
     const uint64_t cmix = mix + rdlane64(hs, c);
     if (Pn == 0) {
       clk.to(kBdMaximal);
-      if (Xn == 0) report(pr, hist_lds, t, rbase + d + 1, cmix);
+      if (Xn == 0) keep_clique(pr, hist_lds, t, kb, rbase + d + 1, cmix);
       continue;
     }
     const uint32_t np = popc64(Pn);
@@ -1470,7 +1513,7 @@ __device__ __forceinline__ void solve_small(const BkParams &pr, const Sub &s, ui
       if (np == 1) {
         if ((Xn & Ma) == 0) {
           ++nodes;
-          report(pr, hist_lds, t, rbase + d + 2, cmix + ha);
+          keep_clique(pr, hist_lds, t, kb, rbase + d + 2, cmix + ha);
         }
         continue;
       }
@@ -1479,21 +1522,21 @@ __device__ __forceinline__ void solve_small(const BkParams &pr, const Sub &s, ui
       if (Xn & Ma & Mb) continue;  // an X member covers P
       if ((Ma >> b) & 1ull) {
         nodes += 2;
-        report(pr, hist_lds, t, rbase + d + 3, cmix + ha + hb);
+        keep_clique(pr, hist_lds, t, kb, rbase + d + 3, cmix + ha + hb);
         continue;
       }
       const uint64_t xs = Xn & (Ma | Mb);
       if (xs == 0) {
         nodes += 2;
-        report(pr, hist_lds, t, rbase + d + 2, cmix + ha);
-        report(pr, hist_lds, t, rbase + d + 2, cmix + hb);
+        keep_clique(pr, hist_lds, t, kb, rbase + d + 2, cmix + ha);
+        keep_clique(pr, hist_lds, t, kb, rbase + d + 2, cmix + hb);
         continue;
       }
       // the pivot is the lowest X member adjacent to one of a, b; the child
       // is the other one
       const bool piv_a = (Ma >> ctz64(xs)) & 1ull;
       ++nodes;
-      if ((Xn & (piv_a ? Mb : Ma)) == 0) report(pr, hist_lds, t, rbase + d + 2, cmix + (piv_a ? hb : ha));
+      if ((Xn & (piv_a ? Mb : Ma)) == 0) keep_clique(pr, hist_lds, t, kb, rbase + d + 2, cmix + (piv_a ? hb : ha));
       continue;
     }
     clk.to(kBdPivot);
@@ -1515,10 +1558,11 @@ __device__ __forceinline__ void solve_small(const BkParams &pr, const Sub &s, ui
     piv = npiv;
     Np = rdlane64(M, piv);
   }
+  if (kb.n) flush_cliques(pr, hist_lds, kb);
   wa.nodes += t.nodes;
   wa.count += t.count;
   wa.sub_cliques += t.count;
-  wa.hash += t.hash;
+  wa.hash += wave_add64(kb.hacc);
   wa.max_size = max(wa.max_size, t.max_size);
 }
 
