@@ -228,20 +228,43 @@ def main():
         _, split, _ = dg.download(prio=False, split=True, col=False)
         plain = None
         if not args.no_plain:
-            # the plain-schedule engine (heaviest-first by out-degree, worker
-            # list, default settings) on the same resident graph, with
-            # per-subproblem tree sizes to score the cost model's ranking
-            t = time.perf_counter()
-            dg.order()
-            rp_ = dg.count(level=args.level, induced=args.induced, worklist=2, schedule=0, sub_nodes=True)
-            plain_ms = (time.perf_counter() - t) * 1e3
-            if (rp_["count"], rp_["hash"]) != (first["count"], first["hash"]):
-                raise SystemExit("plain and learned schedules disagree")
+            # The plain-schedule engine on the same resident graph with the same
+            # settings (tuned block settings, variant rule, graph launch, warm-up
+            # and steps): only the schedule changes. Under -w auto the learned
+            # engine picks the variant from its predictions while the plain one
+            # takes the worker list, so when the learned run picked another
+            # variant the plain engine is also timed with that one.
+            pkw = dict(kw, schedule=SCHEDULES["plain"], model_path=None)
+
+            def time_plain(extra):
+                for _ in range(min(args.warmup, 1)):
+                    dg.order()
+                    dg.count(**pkw, **extra)
+                ms_, kern_, last = [], [], None
+                for _ in range(args.steps):
+                    t = time.perf_counter()
+                    dg.order()
+                    last = dg.count(**pkw, **extra)
+                    ms_.append((time.perf_counter() - t) * 1e3)
+                    kern_.append(last["t_count_ms"])
+                    if (last["count"], last["hash"]) != (first["count"], first["hash"]):
+                        raise SystemExit("plain and learned schedules disagree")
+                return float(np.mean(ms_)), float(np.mean(kern_)), last
+
+            plain_ms, plain_kern, rp_ = time_plain({})
             plain = {"value": rp_["count"] / (plain_ms / 1e3), "ms_per_step": plain_ms,
-                     "enum_kernel_ms": rp_["t_count_ms"], "worklist": "wl", "schedule": "plain"}
+                     "enum_kernel_ms": plain_kern, "schedule": "plain",
+                     "worklist": {1: "nowl", 2: "wl"}[rp_["worklist_used"]]}
+            if rp_["worklist_used"] != first["worklist_used"]:
+                sv_ms, sv_kern, _ = time_plain({"worklist": first["worklist_used"]})
+                plain["same_variant"] = {"worklist": {1: "nowl", 2: "wl"}[first["worklist_used"]],
+                                         "ms_per_step": sv_ms, "enum_kernel_ms": sv_kern,
+                                         "speedup_learned": sv_ms / float(np.mean(step_ms))}
+            # per-subproblem tree sizes score the cost model's ranking (untimed)
+            lab = dg.count(**pkw, sub_nodes=True)
             F = dg.features()
             roots = F[:, 1] > 0
-            y = np.log1p(rp_["sub_nodes"].astype(np.float64))[roots]
+            y = np.log1p(lab["sub_nodes"].astype(np.float64))[roots]
             plain["spearman_out_degree"] = spearman(F[roots, 1], y)
             if model_path and os.path.exists(model_path):
                 pred = mce.mlp_predict(mce.mlp_load(model_path), F[roots])
@@ -282,6 +305,7 @@ def main():
         "count": count, "hash": f"{first['hash']:#018x}", "hist": first["hist"], "degeneracy": first["degeneracy"],
         "tree_nodes": first["tree_nodes"], "enum_kernel_ms": kavg, "order_orient_ms": float(np.mean(order_ms)),
         "fallback_events": first["fallback_events"], "waves": first["num_waves"], "donations": first["donations"],
+        "worklist_used": {1: "nowl", 2: "wl"}[first["worklist_used"]],
         "plain_engine": plain, "speedup_vs_plain": (plain["ms_per_step"] / ms) if plain else None,
         "autotune": tuned,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

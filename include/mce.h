@@ -100,7 +100,9 @@ typedef struct mce_opts {
   int groups_per_cu;       /* resident workgroups per CU (1..16)            */
   int lds_arena_bytes;     /* LDS pool bytes per wave; a subproblem takes
                               what its bitmaps need (0 = the CU's LDS)     */
-  int hbm_arena_bytes;     /* HBM spill arena per wave for larger problems  */
+  int hbm_arena_bytes;     /* cap of the HBM arena per wave (0 = 16 MiB);
+                              subproblems that need more than the cap run
+                              in a second launch with larger arenas       */
   int stack_lds_levels;    /* BK frames kept in LDS before spilling to HBM  */
   int donate_min_p;        /* donate a child only if |P| >= this (SZTH)     */
   int donate_check_nodes;  /* tree nodes between polls of the idle count    */
@@ -121,6 +123,11 @@ typedef struct mce_opts {
                               rows number at most this many has its whole
                               subtree solved with one lane per member
                               (1..64; 0 = 64; -1 = never)                   */
+  int watchdog_us;         /* worker list: an idle wave that sees no global
+                              progress (no finished task, no heartbeat of
+                              a busy wave) for this long stops and the call
+                              fails with MCE_EINTERNAL instead of hanging
+                              (0 = 120 s)                                   */
 
   int schedule;            /* enum mce_schedule                              */
   const uint32_t *order;   /* MCE_SCHED_GIVEN: first-level vertex order      */
@@ -152,6 +159,7 @@ typedef struct mce_result {
   uint32_t max_degree;       /* Delta                                   */
   uint32_t max_out_degree;   /* Delta+ after degeneracy orientation     */
   uint32_t num_waves;        /* resident waves of the enumeration launch */
+  uint32_t worklist_used;    /* the variant that ran: MCE_NOWL or MCE_WL  */
   uint64_t tree_nodes;       /* BK tree nodes expanded                  */
   uint64_t donations;        /* worker-list donations                   */
   uint64_t fallback_events;  /* learned schedule fell back to plain     */

@@ -67,7 +67,8 @@ class Opts(ctypes.Structure):
         ("level", c_int), ("induced", c_int), ("worklist", c_int), ("task", c_int), ("device", c_int),
         ("waves_per_group", c_int), ("groups_per_cu", c_int), ("lds_arena_bytes", c_int),
         ("hbm_arena_bytes", c_int), ("stack_lds_levels", c_int), ("donate_min_p", c_int),
-        ("donate_check_nodes", c_int), ("donate_age_us", c_int), ("split_heavy", c_int), ("graph_launch", c_int), ("small_universe", c_int), ("schedule", c_int), ("order", POINTER(c_uint32)),
+        ("donate_check_nodes", c_int), ("donate_age_us", c_int), ("split_heavy", c_int), ("graph_launch", c_int), ("small_universe", c_int),
+        ("watchdog_us", c_int), ("schedule", c_int), ("order", POINTER(c_uint32)),
         ("order_len", c_uint64), ("model_path", c_char_p), ("model", POINTER(Mlp)), ("sub_nodes", POINTER(c_uint64)),
         ("sub_cliques", POINTER(c_uint64)),
     ]
@@ -81,7 +82,7 @@ class Result(ctypes.Structure):
     _fields_ = [
         ("count", c_uint64), ("hash", c_uint64), ("hist", POINTER(c_uint64)), ("hist_len", c_uint32),
         ("max_clique_size", c_uint32), ("degeneracy", c_uint32), ("max_degree", c_uint32),
-        ("max_out_degree", c_uint32), ("num_waves", c_uint32), ("tree_nodes", c_uint64),
+        ("max_out_degree", c_uint32), ("num_waves", c_uint32), ("worklist_used", c_uint32), ("tree_nodes", c_uint64),
         ("donations", c_uint64), ("fallback_events", c_uint64), ("split_vertices", c_uint64), ("t_order_ms", c_double),
         ("t_orient_ms", c_double), ("t_count_ms", c_double), ("wave_stats", POINTER(WaveStat)),
         ("wave_stats_cap", c_uint32), ("wave_stats_len", c_uint32),
@@ -387,7 +388,8 @@ def _result_dict(r: Result, hist: np.ndarray) -> dict:
     return dict(
         count=int(r.count), hash=int(r.hash), hist={int(k): int(hist[k]) for k in nz},
         max_clique_size=int(r.max_clique_size), degeneracy=int(r.degeneracy), max_degree=int(r.max_degree),
-        max_out_degree=int(r.max_out_degree), num_waves=int(r.num_waves), tree_nodes=int(r.tree_nodes),
+        max_out_degree=int(r.max_out_degree), num_waves=int(r.num_waves), worklist_used=int(r.worklist_used),
+        tree_nodes=int(r.tree_nodes),
         donations=int(r.donations), fallback_events=int(r.fallback_events), split_vertices=int(r.split_vertices),
         t_order_ms=float(r.t_order_ms),
         t_orient_ms=float(r.t_orient_ms), t_count_ms=float(r.t_count_ms),

@@ -82,7 +82,7 @@ int check_opts(const mce_opts *o) {
   if (o->waves_per_group < 0 || o->waves_per_group > 12) return set_error(MCE_EINVAL, "waves_per_group must be 0..12");
   if (o->groups_per_cu > 16) return set_error(MCE_EINVAL, "groups_per_cu must be 0..16");
   if (o->groups_per_cu < 0 || o->lds_arena_bytes < 0 || o->hbm_arena_bytes < 0 || o->stack_lds_levels < 0 ||
-      o->donate_min_p < 0 || o->donate_check_nodes < 0 || o->donate_age_us < 0)
+      o->donate_min_p < 0 || o->donate_check_nodes < 0 || o->donate_age_us < 0 || o->watchdog_us < 0)
     return set_error(MCE_EINVAL, "negative block setting");
   if (o->lds_arena_bytes && (o->lds_arena_bytes < 1024 || o->lds_arena_bytes % 8))
     return set_error(MCE_EINVAL, "lds_arena_bytes must be a multiple of 8 and >= 1024");
@@ -415,6 +415,8 @@ static int count_impl(mce_graph *h, const mce_opts *opts, mce_result *res, bool 
   cfg.donate_age_us = opts->donate_age_us ? (uint32_t)opts->donate_age_us : 1000;
   cfg.capture = capture || opts->graph_launch;
   cfg.small_max = opts->small_universe < 0 ? 0u : opts->small_universe ? (uint32_t)opts->small_universe : 64u;
+  cfg.hbm_bytes = (uint64_t)opts->hbm_arena_bytes;
+  cfg.watchdog_us = (uint32_t)opts->watchdog_us;
   if (heavy_tail && !opts->donate_age_us) {
     // thresholds set from the cost model: with a predicted heavy tail only
     // long-running tasks donate (a donation copies the donor's bit matrix;
@@ -449,6 +451,7 @@ static int count_impl(mce_graph *h, const mce_opts *opts, mce_result *res, bool 
   res->donations = out.donations;
   res->max_clique_size = out.max_size;
   res->num_waves = out.waves;
+  res->worklist_used = cfg.wl ? MCE_WL : MCE_NOWL;
   res->t_count_ms = out.ms;
   cleanup();
   return MCE_OK;

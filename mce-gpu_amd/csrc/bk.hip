@@ -46,6 +46,7 @@
 
 #include "engine.hpp"
 #include "internal.hpp"
+#include "layout.hpp"
 #include "rng.hpp"
 #include "wave.hpp"
 
@@ -71,20 +72,22 @@ enum Bd : int { kBdOther = 0, kBdInduced, kBdQueue, kBdPivot, kBdSetOps, kBdMaxi
 // word kCtlQueues + 16q), so that donors and idle waves of different XCDs do
 // not contend for one line.
 constexpr uint32_t kQueues = 8;
-constexpr uint32_t kCtlOutstanding = 16, kCtlQueues = 32, kCtlWords = kCtlQueues + 16 * kQueues;
+// Word kCtlProgress (a line of its own) counts heartbeats of busy waves.
+constexpr uint32_t kCtlProgress = 0, kCtlOutstanding = 16, kCtlQueues = 32, kCtlWords = kCtlQueues + 16 * kQueues;
 // Mailbox words of the waves, each on its own 128-byte line.
 constexpr uint32_t kMboxStride = 16;
 constexpr unsigned long long kNoSlot = ~0ull;
 constexpr uint32_t kSlotHeader = 64;  // bytes
-// IPX subproblems with at most this many outside-X rows keep them as a bitmap
-// over row indices (one word per lane, like P and X); larger ones use lists.
-constexpr uint32_t kMaxXRows = 4096;
 // Most waves in one workgroup (the launch bound of the enumeration kernel).
 constexpr uint32_t kMaxWpg = 12;
 enum TaskKind : uint32_t { kTaskList = 0, kTaskBits = 1 };
-// 100 MHz realtime ticks an idle wave waits without receiving work before it
-// reports a termination failure and exits: 120 s.
+// Idle waves watch for global progress (a change of the outstanding-work
+// count or of the busy waves' heartbeat counter); after this many 100 MHz
+// realtime ticks without any they report a termination failure and exit
+// (default 120 s; BkParams::watchdog_ticks overrides it). A busy wave beats
+// once per kBeatNodes tree nodes.
 constexpr uint64_t kWatchdogTicks = 12000000000ull;
+constexpr uint64_t kBeatNodes = 1u << 16;
 
 struct BkParams {
   const uint64_t *rp;
@@ -131,6 +134,7 @@ struct BkParams {
   uint64_t donate_age;  // 100 MHz realtime ticks a task runs before it may donate
   uint32_t small_max;   // nodes with P + X + outside rows <= this go to the small-universe solver
   uint32_t small_pmax;  // ... and, with the worker list, |P| <= this
+  uint64_t watchdog_ticks;  // idle time without global progress before the kernel gives up
 };
 
 struct Sub {
@@ -162,6 +166,7 @@ struct WaveAcc {
   uint64_t sub_cliques = 0;
   int32_t budget = 0;
   uint64_t last_poll = 0;
+  uint64_t last_beat = 0;  // tree nodes at the last heartbeat
   uint32_t xq = 0;      // this wave's idle queue (its XCD)
   uint32_t peek = 0;    // another queue sampled at the next poll
   uint64_t t_task = 0;  // realtime at the start of the current task
@@ -189,8 +194,6 @@ struct Clock {
   }
 };
 
-__host__ __device__ __forceinline__ uint64_t align8(uint64_t x) { return (x + 7) & ~7ull; }
-__host__ __device__ __forceinline__ uint64_t min64(uint64_t a, uint64_t b) { return a < b ? a : b; }
 
 // All-ones P over the first p bits (lane k holds word k).
 __device__ __forceinline__ uint64_t full_word(uint32_t p) {
@@ -257,37 +260,19 @@ __device__ __forceinline__ uint8_t *frame_at(const Sub &s, uint32_t d) {
   return d < s.lds_levels ? s.fr_lds + (uint64_t)d * s.FS : s.fr_hbm + (uint64_t)(d - s.lds_levels) * s.FS;
 }
 
-// Fixed arena bytes for bounds (p, q, W).
-__host__ __device__ __forceinline__ uint64_t fixed_bytes(uint32_t p, uint32_t q, uint32_t W, bool ipx) {
-  return align8(4ull * p) + 8ull * p * W + (ipx ? 8ull * q * W + align8(4ull * q) : 0) + 2 * align8(4ull * q);
-}
-
-// Lays out the arena for bounds (p, q, W). Returns false when the HBM arena
-// is too small.
+// Lays out the list-path arena for bounds (p, q, W) over an LDS region and
+// the wave's HBM arena. Returns false when the HBM arena is too small.
 __device__ __forceinline__ bool place(Sub &s, uint32_t p, uint32_t q, uint32_t W, bool ipx, uint8_t *lds, uint32_t lds_bytes,
                       uint8_t *hbm, uint64_t hbm_bytes, uint32_t stack_lds = 0) {
+  uint32_t levels = 0;
+  bool in_lds = false;
+  if (list_hbm_need(p, q, W, ipx, lds_bytes, stack_lds, &levels, &in_lds) > hbm_bytes) return false;
   const uint64_t fixed = fixed_bytes(p, q, W, ipx);
-  const uint32_t FS = 16 * W + 16;
-  const uint32_t D = p;
-  uint8_t *base;
-  uint64_t need;
-  if (fixed + FS <= lds_bytes) {
-    base = lds;
-    s.fr_lds = lds + fixed;
-    s.lds_levels = (uint32_t)((lds_bytes - fixed) / FS);
-    if (stack_lds && s.lds_levels > stack_lds) s.lds_levels = stack_lds;
-    s.fr_hbm = hbm;
-    need = D > s.lds_levels ? (uint64_t)(D - s.lds_levels) * FS : 0;
-  } else {
-    base = hbm;
-    s.fr_lds = lds;
-    s.lds_levels = lds_bytes / FS;
-    if (stack_lds && s.lds_levels > stack_lds) s.lds_levels = stack_lds;
-    s.fr_hbm = hbm + fixed;
-    need = fixed + (D > s.lds_levels ? (uint64_t)(D - s.lds_levels) * FS : 0);
-  }
-  if (need > hbm_bytes) return false;
-  s.FS = FS;
+  uint8_t *base = in_lds ? lds : hbm;
+  s.fr_lds = in_lds ? lds + fixed : lds;
+  s.fr_hbm = in_lds ? hbm : hbm + fixed;
+  s.lds_levels = levels;
+  s.FS = list_fs(W);
   uint64_t o = 0;
   s.S = (uint32_t *)(base + o);
   o += align8(4ull * p);
@@ -301,17 +286,6 @@ __device__ __forceinline__ bool place(Sub &s, uint32_t p, uint32_t q, uint32_t W
   o += align8(4ull * q);
   s.XA = (uint32_t *)(base + o);
   return true;
-}
-
-// HBM bytes a subproblem with bounds (p, q) needs beyond the LDS arena.
-__device__ __forceinline__ uint64_t hbm_need(uint32_t p, uint32_t q, bool ipx) {
-  const uint32_t W = (p + 63) / 64;
-  return fixed_bytes(p, q, W, ipx) + (uint64_t)p * (16 * W + 16);
-}
-
-// LDS bytes a list-path subproblem asks the pool for.
-__host__ __device__ __forceinline__ uint32_t list_want(uint32_t p, uint32_t q, uint32_t W, bool ipx, uint32_t levels) {
-  return (uint32_t)min64(fixed_bytes(p, q, W, ipx) + (uint64_t)(16 * W + 16) * min64(p, levels), 0xFFFFFFFFull);
 }
 
 // Workgroup LDS pool: granules of kGranule bytes handed to waves for the
@@ -680,6 +654,14 @@ __device__ __forceinline__ uint32_t queue_len(const BkParams &pr, uint32_t q) {
   return t > h ? (uint32_t)min(t - h, 0x7fffffffull) : 0u;
 }
 
+// Progress signal of a busy wave for the idle waves' watchdog.
+__device__ __forceinline__ void heartbeat(const BkParams &pr, WaveAcc &wa) {
+  if (wa.nodes - wa.last_beat >= kBeatNodes) {
+    wa.last_beat = wa.nodes;
+    if (lane_id() == 0) atomicAdd(pr.ctl + kCtlProgress, 1ull);
+  }
+}
+
 // Idle waves waiting for work: the length of this wave's queue plus one
 // other queue, visited in turn (lane 0 samples).
 __device__ __forceinline__ int32_t donation_budget(const BkParams &pr, WaveAcc &wa) {
@@ -868,6 +850,7 @@ __device__ __forceinline__ void bk_solve(const BkParams &pr, Sub &s, uint64_t cP
     if constexpr (kWL) {
       if (wa.nodes - wa.last_poll >= pr.donate_check) {
         wa.last_poll = wa.nodes;
+        heartbeat(pr, wa);
         clk.to(kBdQueue);
         // only a task that has run for a while gives work away
         wa.budget = __builtin_amdgcn_s_memrealtime() - wa.t_task >= pr.donate_age ? donation_budget(pr, wa) : 0;
@@ -948,60 +931,40 @@ __device__ __forceinline__ void bk_solve(const BkParams &pr, Sub &s, uint64_t cP
 // E (WX words) and {pivot, mix}. Row index order is vertex-id order, so the
 // lowest-index tie-break equals the list path's id tie-break.
 
-__host__ __device__ __forceinline__ uint64_t bits_fixed(uint32_t p, uint32_t q, uint32_t W) {
-  return align8(4ull * p) + 8ull * p * W + 8ull * q * W + 8ull * p;
-}
-
-__host__ __device__ __forceinline__ uint32_t bits_fs(uint32_t W, uint32_t WX) { return 8 * (2 * W + WX) + 16; }
-
 // Lays out a bitmap subproblem over an LDS region of lds_bytes (taken from
-// the workgroup pool) and the wave's HBM arena. In order of preference: all
-// of S, A, the outside rows and the first frames in LDS; S, A and the first
+// the workgroup pool) and the wave's HBM arena, as bits_plan decides: all of
+// S, A, the outside rows and the first frames in LDS; S, A and the first
 // frames in LDS with the outside rows in HBM; everything but the frames the
-// region holds in HBM. Frames past the LDS levels continue in HBM.
+// region holds in HBM. Frames past the LDS levels continue in HBM. The order
+// S | A | rows | H keeps a level-2 staging list that starts right after the
+// rows intact while the rows are built (H is written last).
 __device__ __forceinline__ bool place_bits(Sub &s, uint32_t p, uint32_t q, uint32_t W, uint8_t *lds, uint32_t lds_bytes,
                                            uint8_t *hbm, uint64_t hbm_bytes, uint32_t stack_lds = 0) {
-  // layout: S | A | (outside rows) | H | frames; H after the rows, so that
-  // a level-2 staging list right after the rows stays intact while the rows
-  // are built (H is written last)
-  const uint32_t WX = (q + 63) / 64;
-  const uint64_t sa = align8(4ull * p) + 8ull * p * W, xr = 8ull * q * W, hb = 8ull * p;
-  const uint32_t FS = bits_fs(W, WX);
-  uint64_t need, in_lds;
-  uint8_t *xbase, *hbase;
-  uint8_t *base;
-  if (sa + xr + hb + FS <= lds_bytes) {
+  const BitsPlan b = bits_plan(p, q, W, lds_bytes, stack_lds);
+  if (b.hbm > hbm_bytes) return false;
+  const uint64_t sa = align8(4ull * p) + 8ull * p * W, xr = 8ull * q * W;
+  uint8_t *base, *xbase, *hbase;
+  if (b.where == 3) {
     base = lds;
     xbase = lds + sa;
     hbase = lds + sa + xr;
-    in_lds = sa + xr + hb;
     s.fr_hbm = hbm;
-    need = 0;
-    s.where = 3;
-  } else if (sa + hb + FS <= lds_bytes) {
+  } else if (b.where == 1) {
     base = lds;
     hbase = lds + sa;
     xbase = hbm;
-    in_lds = sa + hb;
     s.fr_hbm = hbm + xr;
-    need = xr;
-    s.where = 1;
   } else {
     base = hbm;
     xbase = hbm + sa;
     hbase = hbm + sa + xr;
-    in_lds = 0;
-    s.fr_hbm = hbm + sa + xr + hb;
-    need = sa + xr + hb;
-    s.where = 0;
+    s.fr_hbm = hbm + sa + xr + 8ull * p;
   }
-  s.fr_lds = lds + in_lds;
-  s.lds_levels = (uint32_t)((lds_bytes - in_lds) / FS);
-  if (stack_lds && s.lds_levels > stack_lds) s.lds_levels = stack_lds;
-  need += p > s.lds_levels ? (uint64_t)(p - s.lds_levels) * FS : 0;
-  if (need > hbm_bytes) return false;
-  s.FS = FS;
-  s.WX = WX;
+  s.where = b.where;
+  s.fr_lds = lds + b.in_lds;
+  s.lds_levels = b.lds_levels;
+  s.FS = bits_fs(W, words_of(q));
+  s.WX = words_of(q);
   s.S = (uint32_t *)base;
   s.A = (uint64_t *)(base + align8(4ull * p));
   s.XR = (uint64_t *)xbase;
@@ -1010,26 +973,6 @@ __device__ __forceinline__ bool place_bits(Sub &s, uint32_t p, uint32_t q, uint3
   s.XA = nullptr;
   s.XV = nullptr;
   return true;
-}
-
-// LDS bytes a bitmap subproblem asks the pool for: the whole layout plus
-// `levels` frames.
-__host__ __device__ __forceinline__ uint32_t bits_want(uint32_t p, uint32_t q, uint32_t W, uint32_t levels) {
-  const uint32_t WX = (q + 63) / 64;
-  return (uint32_t)min64(bits_fixed(p, q, W) + (uint64_t)bits_fs(W, WX) * min64(p, levels), 0xFFFFFFFFull);
-}
-
-// ... with the outside rows left in HBM.
-__host__ __device__ __forceinline__ uint32_t bits_want_inner(uint32_t p, uint32_t q, uint32_t W, uint32_t levels) {
-  const uint32_t WX = (q + 63) / 64;
-  return (uint32_t)min64(align8(4ull * p) + 8ull * p * W + 8ull * p + (uint64_t)bits_fs(W, WX) * min64(p, levels),
-                         0xFFFFFFFFull);
-}
-
-// HBM arena bytes a bitmap subproblem needs when the pool gives it no LDS.
-__device__ __forceinline__ uint64_t bits_need(uint32_t p, uint32_t q) {
-  const uint32_t W = (p + 63) / 64, WX = (q + 63) / 64;
-  return bits_fixed(p, q, W) + (uint64_t)p * bits_fs(W, WX);
 }
 
 // H[i] = splitmix64(S[i]) for the subproblem's p vertices.
@@ -1588,6 +1531,7 @@ __device__ __forceinline__ void solve_bits_t(const BkParams &pr, Sub &s, uint64_
     if constexpr (kWL) {
       if (wa.nodes - wa.last_poll >= pr.donate_check) {
         wa.last_poll = wa.nodes;
+        heartbeat(pr, wa);
         clk.to(kBdQueue);
         // only a task that has run for a while gives work away
         wa.budget = __builtin_amdgcn_s_memrealtime() - wa.t_task >= pr.donate_age ? donation_budget(pr, wa) : 0;
@@ -1968,6 +1912,7 @@ __global__ __launch_bounds__(kMaxWpg * kWave, 4) void bk_kernel(BkParams pr) {
     Region rsv{pl.base, 0, 0, 0};
     uint32_t backoff = 1, polls = 0;
     uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    unsigned long long seen_out = ~0ull, seen_beat = ~0ull;
     const uint8_t *slot = pr.ring + (uint64_t)gw * pr.slot_bytes;
     while (true) {
       clk.to(kBdQueue);
@@ -2099,8 +2044,15 @@ __global__ __launch_bounds__(kMaxWpg * kWave, 4) void bk_kernel(BkParams pr) {
       if ((++polls & 15) == 0 || !registered) {
         const unsigned long long out = uniform64(L == 0 ? ld_relaxed(pr.ctl + kCtlOutstanding) : 0);
         if (out == 0) break;
+        const unsigned long long beat = uniform64(L == 0 ? ld_relaxed(pr.ctl + kCtlProgress) : 0);
+        if (out != seen_out || beat != seen_beat) {
+          // some wave made progress: the wait starts over
+          seen_out = out;
+          seen_beat = beat;
+          t0 = __builtin_amdgcn_s_memrealtime();
+        }
       }
-      if (__builtin_amdgcn_s_memrealtime() - t0 > kWatchdogTicks) {
+      if (__builtin_amdgcn_s_memrealtime() - t0 > pr.watchdog_ticks) {
         if (L == 0) atomicOr(&pr.acc[kAccError], 1ull);
         break;
       }
@@ -2168,15 +2120,7 @@ __global__ void need_kernel(const uint64_t *rp, const uint64_t *split, const voi
         q = (uint32_t)min(rp[a.x + 1] - rp[a.x], split[a.y] - rp[a.y]);
       }
     }
-    // level-2 items stage their intersections in the list layout of the
-    // bounds, then re-lay for the actual sizes (up to four more HBM frames)
-    uint64_t need;
-    // (sizes for a subproblem the LDS pool gives nothing)
-    if (arc)
-      need = max(hbm_need(p, q, ipx), ipx && q <= kMaxXRows ? bits_need(p, q) : 0ull) +
-             4ull * (8 * (2 * ((p + 63) / 64) + (q + 63) / 64) + 16);
-    else
-      need = ipx && q <= kMaxXRows ? bits_need(p, q) : hbm_need(p, q, ipx);
+    const uint64_t need = item_need(p, q, arc, ipx);
     mx = max(mx, (unsigned long long)need);
     over += need > cap;
   }
@@ -2299,7 +2243,9 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   if ((rc = dmalloc((void **)&ticket, sizeof(uint32_t)))) return rc;
   unsigned long long *need = acc + kAccN;
   MCE_HIP(hipMemsetAsync(acc, 0, sizeof(unsigned long long) * (kAccN + 2), s));
-  const uint64_t cap = 16ull << 20;
+  // per-wave HBM arena cap: items that need more are deferred to a second
+  // pass with arenas sized for the largest
+  const uint64_t cap = cfg.hbm_bytes ? cfg.hbm_bytes : 16ull << 20;
   need_kernel<<<1024, 256, 0, s>>>(g.rp, g.split, d_items, n_items, (uint32_t)cfg.level, cfg.ipx, cap, need);
   unsigned long long hneed[2];
   MCE_HIP(hipMemcpyAsync(hneed, need, sizeof(hneed), hipMemcpyDeviceToHost, s));
@@ -2317,6 +2263,30 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   MCE_HIP(hipMemGetInfo(&freeb, &totalb));
   if (arena && (uint64_t)groups * wpg * arena > freeb / 4)
     groups = (uint32_t)std::max<uint64_t>(1, (freeb / 4) / ((uint64_t)wpg * arena));
+  // Worker list: one task slot per wave. A slot holds a donor's vertex set
+  // and bit matrix, the child's P/X words (and the 64-word outside-row
+  // bitmap) and its outside-X ids (and rows with IPX). The outside-X capacity
+  // covers the largest in-degree when the ring fits a quarter of the free
+  // memory, else what that budget leaves (at least 64: a child with more
+  // outside X is then not donated); when even 64 do not fit, fewer waves run.
+  uint64_t slot_fixed = 0, slot_per_x = 0, slot_xcap = 0;
+  uint32_t slot_pmax = 0, slot_wmax = 0;
+  if (cfg.wl) {
+    slot_pmax = std::max<uint32_t>(info.max_out_degree, 1);
+    slot_wmax = (slot_pmax + 63) / 64;
+    slot_fixed = kSlotHeader + 8ull * (2 * slot_wmax + 64) + align8(4ull * slot_pmax) + 8ull * slot_pmax * slot_wmax;
+    slot_per_x = 4ull + (cfg.ipx ? 8ull * slot_wmax : 0);
+    const uint64_t want_x = std::max<uint64_t>(info.max_in_degree, 64);
+    const uint64_t budget = freeb / 4;
+    const uint64_t min_slot = slot_fixed + 64 * slot_per_x + 512;
+    if ((uint64_t)groups * wpg * min_slot > budget) {
+      groups = (uint32_t)(budget / ((uint64_t)wpg * min_slot));
+      if (!groups) return set_error(MCE_ENOMEM, "worker-list task slots of %llu bytes do not fit in device memory",
+                                    (unsigned long long)min_slot);
+    }
+    const uint64_t per_slot = budget / ((uint64_t)groups * wpg);
+    slot_xcap = std::min<uint64_t>(want_x, (per_slot - slot_fixed - 512) / slot_per_x);
+  }
   const uint32_t waves = groups * wpg;
 
   uint8_t *hbm = nullptr;
@@ -2346,6 +2316,7 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   pr.want_levels = cfg.stack_lds_levels ? cfg.stack_lds_levels : 24;
   pr.small_max = std::min<uint32_t>(cfg.small_max, kSmallMax);
   pr.small_pmax = kSmallMaxPWl;
+  pr.watchdog_ticks = cfg.watchdog_us ? (uint64_t)cfg.watchdog_us * 100 : kWatchdogTicks;
   pr.hbm = hbm;
   pr.hbm_bytes = arena;
   pr.acc = acc;
@@ -2363,21 +2334,14 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   const uint64_t n_tickets = n_items <= pr.t1 ? n_items : pr.t1 + (n_items - pr.t1 + pr.chunk - 1) / pr.chunk;
   if (n_tickets >= 0xFFFFFF00ull) return set_error(MCE_ELIMIT, "too many work tickets");
   if (cfg.wl) {
-    // A slot holds a donor's vertex set and bit matrix, the child's P/X words
-    // and its outside-X ids (and rows with IPX). One slot per wave, within a
-    // 2 GiB budget; the outside-X capacity takes what the matrix leaves.
-    auto a8 = [](uint64_t x) { return (x + 7) & ~7ull; };
-    pr.slot_pmax = std::max<uint32_t>(info.max_out_degree, 1);
-    pr.slot_wmax = (pr.slot_pmax + 63) / 64;
+    pr.slot_pmax = slot_pmax;
+    pr.slot_wmax = slot_wmax;
     pr.slot_oP = kSlotHeader;
     pr.slot_oS = pr.slot_oP + 8 * (2 * pr.slot_wmax + 64);  // P, X, and the 64-word outside-X bitmap
-    pr.slot_oA = pr.slot_oS + (uint32_t)a8(4ull * pr.slot_pmax);
-    const uint64_t fixed = pr.slot_oA + 8ull * pr.slot_pmax * pr.slot_wmax;
-    const uint64_t per_x = 4ull + (cfg.ipx ? 8ull * pr.slot_wmax : 0);
-    const uint64_t budget = std::max<uint64_t>(fixed + 64 * per_x, (2ull << 30) / std::max<uint32_t>(waves, 1));
-    pr.slot_xcap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(info.max_in_degree, 64), (budget - fixed) / per_x);
-    pr.slot_oXI = (uint32_t)fixed;
-    pr.slot_oXR = (uint32_t)(fixed + a8(4ull * pr.slot_xcap));
+    pr.slot_oA = pr.slot_oS + (uint32_t)align8(4ull * pr.slot_pmax);
+    pr.slot_xcap = (uint32_t)slot_xcap;
+    pr.slot_oXI = (uint32_t)slot_fixed;
+    pr.slot_oXR = (uint32_t)(slot_fixed + align8(4ull * pr.slot_xcap));
     pr.slot_bytes = ((uint64_t)pr.slot_oXR + (cfg.ipx ? 8ull * pr.slot_xcap * pr.slot_wmax : 0) + 255) & ~255ull;
     pr.ring_cap = waves;
     pr.donate_min_p = std::max<uint32_t>(cfg.donate_min_p, 1);
@@ -2422,13 +2386,14 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
     MCE_HIP(hipEventRecord(e0, s));
     if ((rc = dispatch(cfg.ipx, cfg.wl, stat, false, nullptr, pr, groups, wpg, lds, s))) return rc;
   }
-  // items that did not fit: rerun them with arenas sized for the largest
+  // items that did not fit: rerun them with arenas sized for the largest.
+  // The deferred count is always read: an item deferred beyond what
+  // need_kernel predicted would otherwise be lost without an error.
   unsigned long long novf = 0;
-  if (hneed[1]) {
-    MCE_HIP(hipMemcpyAsync(&novf, &acc[kAccOverflow], sizeof(novf), hipMemcpyDeviceToHost, s));
-    MCE_HIP(hipStreamSynchronize(s));
-  }
-  if (novf > hneed[1]) return set_error(MCE_EINTERNAL, "arena overflow list overrun (%llu > %llu)", novf, hneed[1]);
+  MCE_HIP(hipMemcpyAsync(&novf, &acc[kAccOverflow], sizeof(novf), hipMemcpyDeviceToHost, s));
+  MCE_HIP(hipStreamSynchronize(s));
+  if (novf > hneed[1])
+    return set_error(MCE_EINTERNAL, "%llu items did not fit their arena, %llu predicted", novf, hneed[1]);
   if (novf) {
     const uint64_t big = (hneed[0] + 255) & ~255ull;
     const uint32_t g2 = (uint32_t)std::max<uint64_t>(

@@ -49,6 +49,8 @@ void usage() {
           "\n    --donate-check N      tree nodes between idle-worker polls"
           "\n    --donate-age US       a task donates only after running this long"
           "\n    --small-universe N    nodes with |P|+|X| <= N (1..64) run one lane per member; -1 = never"
+          "\n    --hbm-arena BYTES     HBM arena cap per wave (default 16 MiB; larger subproblems run in a second launch)"
+          "\n    --watchdog-us US      worker list: give up after this long without progress (default 120 s)"
           "\n    --schedule plain|learned   first-level order (default plain)"
           "\n    --model PATH          cost model for --schedule learned"
           "\n"
@@ -100,6 +102,8 @@ int main(int argc, char **argv) {
                                      {"model", required_argument, 0, 8},
                                      {"donate-age", required_argument, 0, 9},
                                      {"small-universe", required_argument, 0, 10},
+                                     {"hbm-arena", required_argument, 0, 11},
+                                     {"watchdog-us", required_argument, 0, 12},
                                      {0, 0, 0, 0}};
   int opt;
   while ((opt = getopt_long(argc, argv, "g:r:d:m:p:i:w:h", longopts, nullptr)) >= 0) {
@@ -159,6 +163,8 @@ int main(int argc, char **argv) {
       case 8: model = optarg; break;
       case 9: opts.donate_age_us = atoi(optarg); break;
       case 10: opts.small_universe = atoi(optarg); break;
+      case 11: opts.hbm_arena_bytes = atoi(optarg); break;
+      case 12: opts.watchdog_us = atoi(optarg); break;
       default:
         fprintf(stderr, "\nUnrecognized option!\n");
         usage();

@@ -56,6 +56,7 @@ struct BkLaunch {
   uint32_t donate_check_nodes = 64;
   uint32_t donate_age_us = 1000;
   uint32_t small_max = 64;  // P + X + outside rows at which a node goes to the small-universe solver (0 = never)
+  uint32_t watchdog_us = 0;  // worker list: idle time without global progress before giving up (0 = 120 s)
   bool capture = false;  // run the enumeration launch as a captured hipGraph
 };
 

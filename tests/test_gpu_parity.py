@@ -30,6 +30,8 @@ def run_all(mce, oracle, g, what, nodes=True, **block):
             ref = oracle.mce(g, level=lvl, ipx=(ind == 2))
             res = dg.count(level=lvl, induced=ind, **block)
             assert info["degeneracy"] == ref["degeneracy"], what
+            # maximum degree and out-degree after orientation (Delta, Delta+)
+            assert (info["max_degree"], info["max_out_degree"]) == (ref["max_degree"], ref["max_out_degree"]), what
             assert_same(res, ref, (what, lvl, ind), nodes=nodes)
 
 
@@ -59,6 +61,28 @@ def test_er_4096_config(mce, oracle):
     """Erdős–Rényi n=4096, m=65,536, seed 1 (the small plain-method config)."""
     g = mce.gen_er(4096, 65536, 1)
     run_all(mce, oracle, g, "er4096")
+
+
+def test_rmat_scale14(mce, oracle):
+    """R-MAT scale 14 (127,583,041 maximal cliques at seed 3): level 1 with and
+    without the worker list, level 2, against the oracle."""
+    g = mce.gen_rmat(14, 16, seed=3)
+    with mce.DeviceGraph(g) as dg:
+        dg.order()
+        for lvl, wl in ((1, 1), (1, 2), (2, 2)):
+            ref = oracle.mce(g, level=lvl, ipx=True)
+            print(f"case rmat14 level={lvl} worklist={wl}", flush=True)
+            res = dg.count(level=lvl, induced=2, worklist=wl)
+            assert_same(res, ref, ("rmat14", lvl, wl))
+
+
+@pytest.mark.parametrize("wl", [1, 2])
+def test_deferred_items_second_pass(mce, oracle, wl):
+    """A small LDS pool and a 4 KiB HBM arena cap defer the larger subproblems
+    to the second launch with arenas sized for the largest; results and tree
+    nodes stay those of the oracle in every variant."""
+    g = rand_graph(mce, 400, 0.3, 21)
+    run_all(mce, oracle, g, f"deferred-wl{wl}", worklist=wl, lds_arena_bytes=1024, hbm_arena_bytes=4096)
 
 
 def test_rmat_scale12(mce, oracle):
