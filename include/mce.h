@@ -261,6 +261,24 @@ int mce_graph_order(mce_graph *g, mce_result *res);
 /* Enumerates maximal cliques of an ordered graph. Replaces
  * MultiGPU_MCE::mce_count (mce/mce.cuh:107-415) + sync + show (:417-422). */
 int mce_graph_count(mce_graph *g, const mce_opts *opts, mce_result *res);
+/* Allocation-free count. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code that must not call hipMalloc,
+ * such as tight serving loops and harnesses that forbid it). Same results as
+ * mce_graph_count with level 1, no worker list and the plain schedule (count,
+ * hash, histogram and tree nodes), for opts->induced IP or IPX and the block
+ * settings; other levels, variants, tasks, schedules, graph_launch and the
+ * per-subproblem outputs are refused with MCE_EINVAL. Nothing is allocated:
+ * the work list, counters, histogram and per-wave arenas come from one
+ * static device array of MCE_NOALLOC_SCRATCH_BYTES that each GPU the library
+ * runs on holds until the process ends. The work list takes 4 bytes per
+ * vertex; when the arenas of all resident waves do not fit in what is left,
+ * fewer waves run; a graph whose work list, or a subproblem whose arena, does
+ * not fit fails with MCE_ELIMIT. Every call on a GPU shares that GPU's array
+ * (it keeps the last call's contents, and each call clears what must start
+ * at zero): the library serialises such calls with a lock per device, and
+ * callers that bypass it race. */
+#define MCE_NOALLOC_SCRATCH_BYTES (64u << 20)
+int mce_graph_count_noalloc(mce_graph *g, const mce_opts *opts, mce_result *res);
 /* Copies the ordering back: priority[n], split[n] (start of the later
  * neighbours in each row) and the oriented col_ind[m]. Any may be NULL. */
 int mce_graph_download(const mce_graph *g, uint32_t *priority,

@@ -111,6 +111,7 @@ SIGNATURES = {
     "mce_graph_create": (c_int, [POINTER(Csr), c_int, POINTER(c_void_p)]),
     "mce_graph_order": (c_int, [c_void_p, POINTER(Result)]),
     "mce_graph_count": (c_int, [c_void_p, POINTER(Opts), POINTER(Result)]),
+    "mce_graph_count_noalloc": (c_int, [c_void_p, POINTER(Opts), POINTER(Result)]),
     "mce_graph_download": (c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint64), POINTER(c_uint32)]),
     "mce_graph_destroy": (None, [c_void_p]),
     "mce_graph_features": (c_int, [c_void_p, POINTER(ctypes.c_float)]),
@@ -353,6 +354,17 @@ class DeviceGraph:
         if ws is not None:
             out["wave_stats"] = [(ws[i].wave, ws[i].cu, ws[i].value, tuple(ws[i].bd)) for i in range(r.wave_stats_len)]
         return out
+
+    def count_noalloc(self, induced=IPX, **block) -> dict:
+        """mce_graph_count_noalloc: level 1, no worker list, plain schedule,
+        with every device buffer taken from the library's static scratch array."""
+        o = default_opts(level=L1, induced=induced, worklist=NOWL, device=self.device, **block)
+        r = Result()
+        hist = np.zeros(HIST_LEN, np.uint64)
+        r.hist = _u64p(hist)
+        r.hist_len = HIST_LEN
+        _check(lib.mce_graph_count_noalloc(self.handle, byref(o), byref(r)))
+        return _result_dict(r, hist)
 
     def autotune(self, sample_fraction=0.05, level=L1, induced=IPX, worklist=WL, **block):
         """Fastest block settings on a stride sample; returns (settings, trials)."""

@@ -17,14 +17,6 @@
 #include "internal.hpp"
 #include "wave.hpp"
 
-struct mce_graph {
-  int device = 0;
-  hipStream_t s = nullptr;
-  mce::DevGraph g;
-  mce::OrderInfo info;
-  bool ordered = false;
-};
-
 namespace mce {
 namespace {
 
@@ -68,6 +60,8 @@ void fill_order_fields(const mce_graph *h, mce_result *res) {
   res->max_out_degree = h->info.max_out_degree;
 }
 
+}  // namespace
+
 int check_opts(const mce_opts *o) {
   if (o->level != MCE_L1 && o->level != MCE_L2) return set_error(MCE_EINVAL, "level must be MCE_L1 or MCE_L2");
   if (o->induced != MCE_IP && o->induced != MCE_IPX) return set_error(MCE_EINVAL, "induced must be MCE_IP or MCE_IPX");
@@ -94,7 +88,6 @@ int check_opts(const mce_opts *o) {
   return MCE_OK;
 }
 
-}  // namespace
 }  // namespace mce
 
 using namespace mce;

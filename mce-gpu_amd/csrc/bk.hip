@@ -42,6 +42,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <vector>
 
 #include "engine.hpp"
@@ -2185,9 +2186,12 @@ int stat_of(int task) {
 
 int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64_t n_items, const BkLaunch &cfg,
            uint64_t *d_hist, uint32_t hist_len, uint64_t *d_sub_nodes, uint64_t *d_sub_cliques,
-           mce_wave_stat *wave_stats, uint32_t wave_stats_cap, uint32_t *wave_stats_len, hipStream_t s, BkOut *out) {
+           mce_wave_stat *wave_stats, uint32_t wave_stats_cap, uint32_t *wave_stats_len, hipStream_t s, BkOut *out,
+           Scratch *sc) {
   *out = BkOut{};
   if (wave_stats_len) *wave_stats_len = 0;
+  if (sc && (cfg.wl || wave_stats || cfg.capture))
+    return set_error(MCE_EINVAL, "the allocation-free launch has no worker list, per-wave statistics or capture");
   if (n_items == 0) return MCE_OK;
   if (info.max_out_degree > MCE_MAX_OUT_DEGREE)
     return set_error(MCE_ELIMIT, "out-degree %u exceeds the engine limit %u", info.max_out_degree,
@@ -2221,9 +2225,15 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   const size_t lds = hist_bytes + kPoolHeader + pool;
   if (lds > lds_cu) return set_error(MCE_EINVAL, "LDS per workgroup %zu exceeds 160 KiB", lds);
 
-  // every device buffer of this call, released on all paths
+  // every device buffer of this call, released on all paths (or carved from
+  // the caller's scratch region)
   std::vector<void *> bufs;
   auto dmalloc = [&](void **p, size_t bytes) -> int {
+    if (sc) {
+      *p = sc->take(std::max<size_t>(bytes, 1));
+      if (!*p) return set_error(MCE_ELIMIT, "scratch region too small: %zu more bytes needed", bytes);
+      return MCE_OK;
+    }
     if (hipMalloc(p, std::max<size_t>(bytes, 1)) != hipSuccess)
       return set_error(MCE_ENOMEM, "device allocation of %zu bytes failed", bytes);
     bufs.push_back(*p);
@@ -2244,12 +2254,18 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   unsigned long long *need = acc + kAccN;
   MCE_HIP(hipMemsetAsync(acc, 0, sizeof(unsigned long long) * (kAccN + 2), s));
   // per-wave HBM arena cap: items that need more are deferred to a second
-  // pass with arenas sized for the largest
-  const uint64_t cap = cfg.hbm_bytes ? cfg.hbm_bytes : 16ull << 20;
+  // pass with arenas sized for the largest. From a scratch region, one
+  // workgroup's arenas must fit what is left of it, and there is no second
+  // pass.
+  uint64_t cap = cfg.hbm_bytes ? cfg.hbm_bytes : 16ull << 20;
+  if (sc) cap = std::min<uint64_t>(cap, (sc->left() / wpg) & ~255ull);
   need_kernel<<<1024, 256, 0, s>>>(g.rp, g.split, d_items, n_items, (uint32_t)cfg.level, cfg.ipx, cap, need);
   unsigned long long hneed[2];
   MCE_HIP(hipMemcpyAsync(hneed, need, sizeof(hneed), hipMemcpyDeviceToHost, s));
   MCE_HIP(hipStreamSynchronize(s));
+  if (sc && hneed[1])
+    return set_error(MCE_ELIMIT, "%llu subproblems need more than the %llu scratch bytes per wave left (largest: %llu)",
+                     (unsigned long long)hneed[1], (unsigned long long)cap, (unsigned long long)hneed[0]);
 
   int nb = 0;
   BkParams pr{};
@@ -2260,7 +2276,8 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   uint64_t arena = std::min<uint64_t>(hneed[0], cap);
   arena = (arena + 255) & ~255ull;
   size_t freeb = 0, totalb = 0;
-  MCE_HIP(hipMemGetInfo(&freeb, &totalb));
+  if (sc) freeb = 4 * sc->left();  // the arenas may take all that is left
+  else MCE_HIP(hipMemGetInfo(&freeb, &totalb));
   if (arena && (uint64_t)groups * wpg * arena > freeb / 4)
     groups = (uint32_t)std::max<uint64_t>(1, (freeb / 4) / ((uint64_t)wpg * arena));
   // Worker list: one task slot per wave. A slot holds a donor's vertex set
@@ -2355,9 +2372,14 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
     if (!cfg.capture) ring_init<<<64, 256, 0, s>>>(pr.seq, pr.ring_cap, pr.ctl, n_tickets, pr.mbox);
   }
 
-  hipEvent_t e0, e1;
-  MCE_HIP(hipEventCreate(&e0));
-  MCE_HIP(hipEventCreate(&e1));
+  // device-time events, except from a scratch region (event creation may
+  // allocate): then host time around the synchronised launch
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!sc) {
+    MCE_HIP(hipEventCreate(&e0));
+    MCE_HIP(hipEventCreate(&e1));
+  }
+  const auto host_t0 = std::chrono::steady_clock::now();
   MCE_HIP(hipMemsetAsync(ticket, 0, sizeof(uint32_t), s));
   if (cfg.capture) {
     // one persistent launch, captured into a graph: counters reset by a kernel
@@ -2383,7 +2405,7 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
     hipGraphDestroy(graph);
     MCE_HIP(el);
   } else {
-    MCE_HIP(hipEventRecord(e0, s));
+    if (e0) MCE_HIP(hipEventRecord(e0, s));
     if ((rc = dispatch(cfg.ipx, cfg.wl, stat, false, nullptr, pr, groups, wpg, lds, s))) return rc;
   }
   // items that did not fit: rerun them with arenas sized for the largest.
@@ -2427,14 +2449,18 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
     MCE_HIP(hipStreamSynchronize(s));
     if (again != novf) return set_error(MCE_EINTERNAL, "items did not fit the large arena");
   }
-  MCE_HIP(hipEventRecord(e1, s));
+  if (e1) MCE_HIP(hipEventRecord(e1, s));
   unsigned long long h[kAccN];
   MCE_HIP(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, s));
   MCE_HIP(hipStreamSynchronize(s));
   float ms = 0;
-  MCE_HIP(hipEventElapsedTime(&ms, e0, e1));
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  if (e0) {
+    MCE_HIP(hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+  } else {
+    ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - host_t0).count();
+  }
   if (h[kAccError] & 1ull) return set_error(MCE_EINTERNAL, "worker list watchdog expired (termination not reached)");
   if (h[kAccError] & 2ull) return set_error(MCE_EINTERNAL, "a donated subproblem did not fit its receiver");
   out->count = h[kAccCount];

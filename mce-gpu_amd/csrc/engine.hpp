@@ -37,10 +37,30 @@ int peel(const DevGraph &g, hipStream_t s, OrderInfo *info);
 // Orientation into g.oc / g.split / g.dplus plus degree maxima.
 int orient(DevGraph &g, hipStream_t s, OrderInfo *info);
 
+// A fixed device region handed out front to back, 256-byte aligned, in place
+// of hipMalloc (the allocation-free entry point's scratch). Nothing is freed;
+// the owner resets `used`.
+struct Scratch {
+  uint8_t *base = nullptr;
+  uint64_t size = 0;
+  uint64_t used = 0;
+  void *take(uint64_t bytes) {
+    const uint64_t at = (used + 255) & ~255ull;
+    if (at > size || bytes > size - at) return nullptr;
+    used = at + bytes;
+    return base + at;
+  }
+  uint64_t left() const {
+    const uint64_t at = (used + 255) & ~255ull;
+    return at >= size ? 0 : size - at;
+  }
+};
+
 // First-level work list: vertices with out-degree >= 1, heaviest first by
-// `key` (out-degree for the plain schedule). Returns the list length.
+// `key` (out-degree for the plain schedule). Returns the list length. The
+// counting sort's bins come from `sc` when given, else from hipMalloc.
 int schedule_by_key(const DevGraph &g, const uint32_t *key, uint32_t key_max, uint32_t *order,
-                    uint32_t *len_out, hipStream_t s);
+                    uint32_t *len_out, hipStream_t s, Scratch *sc = nullptr);
 
 struct BkLaunch {
   int level = 1;       // 1 = vertex subproblems, 2 = arc subproblems, 3 = both (mixed list)
@@ -77,10 +97,14 @@ int arcs_from_order(const DevGraph &g, const uint32_t *order, uint32_t order_len
 // Enumerates maximal cliques over a first-level work list.
 // level 1: items are vertex ids (uint32); level 2: items are arcs (uint2).
 // hist (device, hist_len entries) is accumulated.
+// With `sc`, every device buffer (counters, per-wave arenas) is carved from
+// that region and nothing is allocated: the worker list, per-wave statistics,
+// graph capture and the second pass for oversized items are then refused,
+// and fewer waves run when the arenas of all of them do not fit.
 int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64_t n_items,
            const BkLaunch &cfg, uint64_t *d_hist, uint32_t hist_len, uint64_t *d_sub_nodes,
            uint64_t *d_sub_cliques, mce_wave_stat *wave_stats, uint32_t wave_stats_cap,
-           uint32_t *wave_stats_len, hipStream_t s, BkOut *out);
+           uint32_t *wave_stats_len, hipStream_t s, BkOut *out, Scratch *sc = nullptr);
 
 int hip_fail(hipError_t e, const char *what);
 
@@ -93,7 +117,19 @@ bool mlp_valid(const mce_mlp *m);
 int learned_keys(const DevGraph &g, const OrderInfo &info, const mce_mlp &m, uint32_t *keys, uint32_t *bad,
                  double *work_sum, double *work_max, hipStream_t s);
 
+// Checks an options block against the ABI's ranges and rules.
+int check_opts(const mce_opts *o);
+
 }  // namespace mce
+
+// Device-resident graph behind the C ABI's opaque handle.
+struct mce_graph {
+  int device = 0;
+  hipStream_t s = nullptr;
+  mce::DevGraph g;
+  mce::OrderInfo info;
+  bool ordered = false;
+};
 
 #define MCE_HIP(call)                                              \
   do {                                                             \

@@ -403,12 +403,17 @@ int orient(DevGraph &g, hipStream_t s, OrderInfo *info) {
 }
 
 int schedule_by_key(const DevGraph &g, const uint32_t *key, uint32_t key_max, uint32_t *order,
-                    uint32_t *len_out, hipStream_t s) {
+                    uint32_t *len_out, hipStream_t s, Scratch *sc) {
   *len_out = 0;
   if (g.n == 0) return MCE_OK;
   uint32_t *buf = nullptr;
   const uint32_t kb = key_max + 1;
-  MCE_HIP(hipMalloc(&buf, sizeof(uint32_t) * (2 * kb + 1)));
+  if (sc) {
+    buf = (uint32_t *)sc->take(sizeof(uint32_t) * (2 * kb + 1));
+    if (!buf) return set_error(MCE_ELIMIT, "scratch too small for the schedule's %u key bins", kb);
+  } else {
+    MCE_HIP(hipMalloc(&buf, sizeof(uint32_t) * (2 * kb + 1)));
+  }
   uint32_t *counts = buf, *cursor = buf + kb, *total = buf + 2 * kb;
   MCE_HIP(hipMemsetAsync(buf, 0, sizeof(uint32_t) * (2 * kb + 1), s));
   const uint32_t tiles = (g.n + kTile - 1) / kTile;
@@ -418,7 +423,7 @@ int schedule_by_key(const DevGraph &g, const uint32_t *key, uint32_t key_max, ui
   sched_scatter<<<tiles, kBlock, lds2, s>>>(g.n, g.dplus, key, key_max, cursor, order);
   MCE_HIP(hipMemcpyAsync(len_out, total, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   MCE_HIP(hipStreamSynchronize(s));
-  hipFree(buf);
+  if (!sc) hipFree(buf);
   return MCE_OK;
 }
 
