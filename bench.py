@@ -79,6 +79,27 @@ def make_workload(mce, name, seed, cl_scale=22):
     raise SystemExit(f"unknown config {name}")
 
 
+def pinned_digest(name, seed, cl_scale, g, off):
+    """Checks the generated workload against the CSR digest committed in
+    tests/golden/configs.json for this configuration and seed, if there is
+    one: a generator change must not silently change the benchmark."""
+    key = {"c3": f"c3_seed{seed}" if cl_scale == 22 else None}.get(name, name if seed == 1 else None)
+    path = os.path.join(ROOT, "tests", "golden", "configs.json")
+    if key is None or not os.path.exists(path):
+        return None
+    fx = json.load(open(path)).get(key)
+    if fx is None or fx["args"].get("seed") != seed:
+        return None
+    got = f"{g.digest():#018x}"
+    if (g.n, g.m, got) != (fx["n"], fx["m"], fx["digest"]):
+        raise SystemExit(f"{name} seed {seed}: CSR digest {got} differs from the committed {fx['digest']}")
+    if off is not None:
+        import hashlib
+        if hashlib.sha256(off.astype(np.uint32).tobytes()).hexdigest() != fx["vtx_off_sha256"]:
+            raise SystemExit(f"{name} seed {seed}: graph offsets differ from the committed ones")
+    return got
+
+
 def sample_workload(mce, g, off, frac):
     """Bounded sample of the same workload. A batch keeps every k-th member
     graph (cliques never cross graphs); a single graph keeps every k-th
@@ -183,6 +204,7 @@ def main():
         fam = {"c3": "cl", "c4": "sbm"}.get(args.config, "sbm")
         args.model = os.path.join(ROOT, "mce-gpu_amd", f"cost_model_{fam}.mlp")
     log(f"[bench] generated {args.config}: n={g.n} arcs={g.m} in {time.time() - t0:.1f}s")
+    desc["digest"] = pinned_digest(args.config, args.seed, args.cl_scale, g, off)
 
     model_path = args.model if args.schedule == "learned" else None
     kw = dict(level=args.level, induced=args.induced, worklist=args.worklist,

@@ -79,8 +79,15 @@ constexpr uint32_t kCtlProgress = 0, kCtlOutstanding = 16, kCtlQueues = 32, kCtl
 constexpr uint32_t kMboxStride = 16;
 constexpr unsigned long long kNoSlot = ~0ull;
 constexpr uint32_t kSlotHeader = 64;  // bytes
-// Most waves in one workgroup (the launch bound of the enumeration kernel).
-constexpr uint32_t kMaxWpg = 12;
+// Launch shape of the enumeration kernel per variant. The worker-list
+// variants carry the most live state (task slots, donation, the idle loop):
+// at 4 waves per SIMD (128 VGPRs) they spill about 520 bytes per lane, so
+// they run at 2 waves per SIMD (256 VGPRs) in workgroups of up to 8 waves
+// (Chung-Lu 2^18: 356 -> 202 ms). The others keep 4 waves per SIMD in
+// workgroups of up to 12: there occupancy is worth more than the spills
+// (32-graph SBM sample: 171 ms at 4, 196 ms at 3, 271 ms at 2).
+constexpr uint32_t max_wpg_of(bool wl) { return wl ? 8u : 12u; }
+constexpr uint32_t waves_per_simd_of(bool wl) { return wl ? 2u : 4u; }
 enum TaskKind : uint32_t { kTaskList = 0, kTaskBits = 1 };
 // Idle waves watch for global progress (a change of the outstanding-work
 // count or of the busy waves' heartbeat counter); after this many 100 MHz
@@ -1749,7 +1756,7 @@ __device__ __forceinline__ void finish_bits(const BkParams &pr, Sub &s, const ui
 template <bool kIPX, bool kWL, int kStat>
 // At least four waves per SIMD (<= 128 VGPRs): occupancy hides the LDS and
 // memory latency of the set operations better than registers would.
-__global__ __launch_bounds__(kMaxWpg * kWave, 4) void bk_kernel(BkParams pr) {
+__global__ __launch_bounds__(max_wpg_of(kWL) * kWave, waves_per_simd_of(kWL)) void bk_kernel(BkParams pr) {
   extern __shared__ __align__(16) uint8_t smem[];
   unsigned long long *hist_lds = (unsigned long long *)smem;
   const uint32_t wig = threadIdx.x / kWave, L = lane_id();
@@ -2207,6 +2214,7 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   // Waves per workgroup: by default all the waves the registers let one CU
   // hold, in as few workgroups as the launch bound allows, so that one LDS
   // pool serves the whole CU.
+  const uint32_t max_wpg = max_wpg_of(cfg.wl);
   uint32_t wpg = cfg.waves_per_group;
   uint32_t groups_cu = std::max<uint32_t>(cfg.groups_per_cu, 1);
   if (!wpg) {
@@ -2215,10 +2223,10 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
     int rc0 = dispatch(cfg.ipx, cfg.wl, stat, true, &nb_regs, probe, 0, 4, hist_bytes + kPoolHeader + 4096, s);
     if (rc0) return rc0;
     const uint32_t waves_cu = 4 * (uint32_t)std::max(nb_regs, 1);
-    groups_cu = std::max<uint32_t>(groups_cu, (waves_cu + kMaxWpg - 1) / kMaxWpg);
+    groups_cu = std::max<uint32_t>(groups_cu, (waves_cu + max_wpg - 1) / max_wpg);
     wpg = std::max<uint32_t>(1, waves_cu / groups_cu);
   }
-  wpg = std::max<uint32_t>(1, std::min<uint32_t>(wpg, kMaxWpg));
+  wpg = std::max<uint32_t>(1, std::min<uint32_t>(wpg, max_wpg));
   size_t pool = cfg.lds_bytes ? (size_t)cfg.lds_bytes * wpg
                               : lds_cu / groups_cu - hist_bytes - kPoolHeader;
   pool = std::min<size_t>(pool, (size_t)kPoolWords * 64 * kGranule) / kGranule * kGranule;

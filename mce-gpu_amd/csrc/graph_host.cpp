@@ -53,9 +53,11 @@ int alloc_csr(uint32_t n, uint64_t m, mce_csr *out) {
   return MCE_OK;
 }
 
-// Counting-sort CSR build: scatter both directions into provisional rows,
-// then sort and deduplicate every row in parallel. The result does not
-// depend on the thread count.
+// CSR build: both directions of every edge are grouped by vertex range
+// (buckets of 2^shift vertices; per-thread counts, so no atomics on hub
+// vertices), each bucket is scattered into provisional rows by one thread,
+// then every row is sorted and deduplicated. The result does not depend on
+// the thread count.
 int build_csr(uint32_t n, const std::vector<uint32_t> &src, const std::vector<uint32_t> &dst,
               mce_csr *out) {
   const uint64_t E = src.size();
@@ -70,35 +72,73 @@ int build_csr(uint32_t n, const std::vector<uint32_t> &src, const std::vector<ui
     }
     n = any ? mx + 1 : 0;
   }
-  std::vector<std::atomic<uint64_t>> deg(n);
-#pragma omp parallel for
-  for (uint64_t v = 0; v < n; ++v) deg[v].store(0, std::memory_order_relaxed);
   bool bad = false;
 #pragma omp parallel for reduction(|| : bad)
-  for (uint64_t i = 0; i < E; ++i) {
-    const uint32_t u = src[i], w = dst[i];
-    if (u >= n || w >= n) {
-      bad = true;
-      continue;
-    }
-    if (u == w) continue;
-    deg[u].fetch_add(1, std::memory_order_relaxed);
-    deg[w].fetch_add(1, std::memory_order_relaxed);
-  }
+  for (uint64_t i = 0; i < E; ++i) bad = bad || src[i] >= n || dst[i] >= n;
   if (bad) return set_error(MCE_EINVAL, "edge endpoint >= n (%u)", n);
-  std::vector<uint64_t> tptr((size_t)n + 1, 0);
-  for (uint64_t v = 0; v < n; ++v) tptr[v + 1] = tptr[v] + deg[v].load(std::memory_order_relaxed);
-  const uint64_t T = tptr[n];
-  std::vector<uint32_t> tcol(std::max<uint64_t>(T, 1));
-#pragma omp parallel for
-  for (uint64_t v = 0; v < n; ++v) deg[v].store(tptr[v], std::memory_order_relaxed);
-#pragma omp parallel for
-  for (uint64_t i = 0; i < E; ++i) {
-    const uint32_t u = src[i], w = dst[i];
-    if (u == w) continue;
-    tcol[deg[u].fetch_add(1, std::memory_order_relaxed)] = w;
-    tcol[deg[w].fetch_add(1, std::memory_order_relaxed)] = u;
+  uint32_t shift = 0;
+  while (((uint64_t)n >> shift) > 16384) ++shift;
+  const uint64_t NB = n ? (((uint64_t)n - 1) >> shift) + 1 : 0;
+  const int T = std::max(omp_get_max_threads(), 1);
+  std::vector<uint64_t> pos((size_t)T * NB + 1, 0);
+  // pass 1: per-thread bucket counts over a static split of the edges
+#pragma omp parallel num_threads(T)
+  {
+    uint64_t *c = pos.data() + (size_t)omp_get_thread_num() * NB;
+#pragma omp for schedule(static)
+    for (uint64_t i = 0; i < E; ++i) {
+      const uint32_t u = src[i], w = dst[i];
+      if (u == w) continue;
+      c[u >> shift]++;
+      c[w >> shift]++;
+    }
   }
+  // bucket-major, thread-minor offsets
+  std::vector<uint64_t> boff(NB + 1, 0);
+  uint64_t T_all = 0;
+  for (uint64_t b = 0; b < NB; ++b) {
+    boff[b] = T_all;
+    for (int t = 0; t < T; ++t) {
+      const uint64_t k = pos[(size_t)t * NB + b];
+      pos[(size_t)t * NB + b] = T_all;
+      T_all += k;
+    }
+  }
+  boff[NB] = T_all;
+  // pass 2: the same static split writes (row, col) pairs at its offsets
+  std::vector<uint32_t> brow(std::max<uint64_t>(T_all, 1)), bcol(std::max<uint64_t>(T_all, 1));
+#pragma omp parallel num_threads(T)
+  {
+    uint64_t *c = pos.data() + (size_t)omp_get_thread_num() * NB;
+#pragma omp for schedule(static)
+    for (uint64_t i = 0; i < E; ++i) {
+      const uint32_t u = src[i], w = dst[i];
+      if (u == w) continue;
+      uint64_t k = c[u >> shift]++;
+      brow[k] = u;
+      bcol[k] = w;
+      k = c[w >> shift]++;
+      brow[k] = w;
+      bcol[k] = u;
+    }
+  }
+  // pass 3: degrees (each vertex belongs to one bucket, so plain counts)
+  std::vector<uint64_t> tptr((size_t)n + 1, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (uint64_t b = 0; b < NB; ++b)
+    for (uint64_t k = boff[b]; k < boff[b + 1]; ++k) tptr[(uint64_t)brow[k] + 1]++;
+  for (uint64_t v = 0; v < n; ++v) tptr[v + 1] += tptr[v];
+  const uint64_t T_rows = tptr[n];
+  std::vector<uint32_t> tcol(std::max<uint64_t>(T_rows, 1));
+  // pass 4: each bucket scatters its pairs into its rows
+#pragma omp parallel for schedule(dynamic, 1)
+  for (uint64_t b = 0; b < NB; ++b) {
+    const uint64_t v0 = b << shift, v1 = std::min<uint64_t>((b + 1) << shift, n);
+    std::vector<uint64_t> cur(tptr.begin() + v0, tptr.begin() + v1);
+    for (uint64_t k = boff[b]; k < boff[b + 1]; ++k) tcol[cur[brow[k] - v0]++] = bcol[k];
+  }
+  std::vector<uint32_t>().swap(brow);
+  std::vector<uint32_t>().swap(bcol);
   std::vector<uint64_t> ulen(n);
 #pragma omp parallel for schedule(dynamic, 1024)
   for (uint64_t v = 0; v < n; ++v) {

@@ -256,6 +256,89 @@ struct Solver {
     }
   }
 
+  // Knuth's random-probe estimate of this subproblem's tree (D. E. Knuth,
+  // "Estimating the efficiency of backtrack programs", Math. Comp. 29, 1975):
+  // one root-to-leaf path, taking at every node one of its k children
+  // uniformly; the product of the k's along the path, times 1 when the path
+  // ends in a maximal clique, is an unbiased estimate of the subtree's
+  // maximal cliques, and sum over the path of (product so far) counts its
+  // nodes the same way. Children are those of expand(): candidate c_i (the
+  // i-th bit of P & ~N(pivot) in index order) sees P without c_1..c_{i-1} and
+  // X with them. Needs build() and the depth-0 frame of run().
+  void probe(uint64_t &rng, double &cl, double &nd) {
+    cl = 0;
+    nd = 0;
+    std::vector<uint64_t> P(Pw(0), Pw(0) + W), X(Xw(0), Xw(0) + W), Pc(W), Xc(W);
+    std::vector<uint32_t> xe = xl[0], xc;
+    uint32_t best, bidx;
+    pivot(P.data(), X.data(), xe, best, bidx);
+    if (best == p) return;
+    double wgt = 1;
+    while (true) {
+      const uint64_t *N = bidx < p ? &A[(uint64_t)bidx * W] : &XR[(uint64_t)(bidx - p) * W];
+      uint32_t k = 0;
+      for (uint32_t j = 0; j < W; ++j) k += __builtin_popcountll(P[j] & ~N[j]);
+      if (k == 0) return;
+      rng = splitmix64(rng);
+      uint32_t pick = (uint32_t)(((unsigned __int128)rng * k) >> 64);
+      wgt *= k;
+      nd += wgt;
+      // move the candidates before the picked one from P to X
+      uint32_t c = 0;
+      for (uint32_t j = 0; j < W; ++j) {
+        uint64_t B = P[j] & ~N[j];
+        while (B) {
+          const uint32_t b = __builtin_ctzll(B);
+          B &= B - 1;
+          if (pick == 0) {
+            c = j * 64 + b;
+            j = W;
+            break;
+          }
+          --pick;
+          P[j] &= ~(1ull << b);
+          X[j] |= 1ull << b;
+        }
+      }
+      P[c >> 6] &= ~(1ull << (c & 63));
+      const uint64_t *Ac = &A[(uint64_t)c * W];
+      bool pany = false, xany = false;
+      uint32_t pc = 0;
+      for (uint32_t j = 0; j < W; ++j) {
+        Pc[j] = P[j] & Ac[j];
+        Xc[j] = X[j] & Ac[j];
+        pany |= Pc[j] != 0;
+        xany |= Xc[j] != 0;
+        pc += __builtin_popcountll(Pc[j]);
+      }
+      xc.clear();
+      for (uint32_t e : xe)
+        if (adj_outside(e, c)) xc.push_back(e);
+      if (!pany) {
+        if (!xany && xc.empty()) cl = wgt;
+        return;
+      }
+      pivot(Pc.data(), Xc.data(), xc, best, bidx);
+      if (best == pc) return;
+      P.swap(Pc);
+      X.swap(Xc);
+      xe.swap(xc);
+    }
+  }
+
+  // Depth-0 frame of run() without the enumeration (for probe()).
+  bool start(const std::vector<uint32_t> &xs) {
+    if (S.empty()) return false;
+    build(xs);
+    bits.assign((uint64_t)2 * (p + 2) * W, 0);
+    if (xl.size() < p + 2) xl.resize(p + 2);
+    uint64_t *P = Pw(0);
+    for (uint32_t i = 0; i < p; ++i) P[i >> 6] |= 1ull << (i & 63);
+    xl[0].resize(q);
+    for (uint32_t e = 0; e < q; ++e) xl[0][e] = ipx ? e : xid[e];
+    return true;
+  }
+
   // Runs BK on (R, P = all of S, X = {} inside S, outside X = xs).
   void run(const std::vector<uint32_t> &xs, uint32_t rsize, uint64_t rmix) {
     nodes = 0;
@@ -539,5 +622,60 @@ uint32_t oracle_order_orient(uint32_t n, const uint64_t *rp, const uint32_t *ci,
 }
 
 int oracle_hist_len(void) { return kHist; }
+
+// Knuth estimates of the level-1 subtrees of the vertices in `order`: the
+// mean over `probes` random probes of each vertex's maximal cliques and tree
+// nodes (Solver::probe), written to est_cliques / est_nodes per entry. The
+// probe random stream of (vertex v, probe j) is splitmix64 of seed, v and j,
+// so results do not depend on the thread count. Returns 0, -1 on bad input,
+// -2 when an out-degree exceeds 4096.
+int oracle_knuth(uint32_t n, const uint64_t *rp, const uint32_t *ci, int ipx, const uint32_t *order,
+                 uint64_t order_len, uint32_t probes, uint64_t seed, int threads, double *est_cliques,
+                 double *est_nodes) {
+  if ((n && (!rp || !ci)) || (order_len && (!order || !est_cliques || !est_nodes)) || probes == 0) return -1;
+  if (threads <= 0) threads = omp_get_max_threads();
+  Graph g;
+  g.n = n;
+  g.rp = rp;
+  g.ci = ci;
+  std::vector<uint32_t> prio(n);
+  if (n) oracle_peel(n, rp, ci, prio.data(), nullptr);
+  g.split.resize(n);
+  g.oc.resize(n ? rp[n] : 0);
+  if (n) oracle_orient(n, rp, ci, prio.data(), g.split.data(), g.oc.data());
+  for (uint32_t v = 0; v < n; ++v)
+    if (rp[v + 1] - g.split[v] > 4096) return -2;
+  for (uint64_t i = 0; i < order_len; ++i)
+    if (order[i] >= n) return -1;
+#pragma omp parallel num_threads(threads)
+  {
+    Acc acc;
+    Solver s;
+    s.g = &g;
+    s.ipx = ipx != 0;
+    s.work = false;
+    s.acc = &acc;
+    std::vector<uint32_t> xs;
+#pragma omp for schedule(dynamic, 1)
+    for (uint64_t i = 0; i < order_len; ++i) {
+      const uint32_t v = order[i];
+      s.S.assign(g.oc.data() + g.split[v], g.oc.data() + rp[v + 1]);
+      xs.assign(g.oc.data() + rp[v], g.oc.data() + g.split[v]);
+      double sc = 0, sn = 0;
+      if (s.start(xs)) {
+        for (uint32_t j = 0; j < probes; ++j) {
+          uint64_t rng = splitmix64(seed ^ splitmix64(((uint64_t)v << 32) | j));
+          double c, nd;
+          s.probe(rng, c, nd);
+          sc += c;
+          sn += nd;
+        }
+      }
+      est_cliques[i] = sc / probes;
+      est_nodes[i] = sn / probes;
+    }
+  }
+  return 0;
+}
 
 }  // extern "C"

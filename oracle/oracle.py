@@ -37,6 +37,9 @@ def _load():
     lib.oracle_peel.argtypes = [c_uint32, c_void_p, c_void_p, c_void_p, c_void_p]
     lib.oracle_core_numbers.restype = c_uint32
     lib.oracle_core_numbers.argtypes = [c_uint32, c_void_p, c_void_p, c_void_p]
+    lib.oracle_knuth.restype = c_int
+    lib.oracle_knuth.argtypes = [c_uint32, c_void_p, c_void_p, c_int, c_void_p, c_uint64, c_uint32, c_uint64, c_int,
+                                 c_void_p, c_void_p]
     lib.oracle_order_orient.restype = c_uint32
     lib.oracle_order_orient.argtypes = [c_uint32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
     return lib
@@ -100,3 +103,17 @@ def order_orient(g):
     rounds = c_uint32()
     d = lib.oracle_order_orient(n, _p(g.row_ptr), _p(g.col_ind), _p(prio), _p(split), _p(oc), byref(rounds))
     return prio[:n], split[:n], oc[:m], int(d)
+
+
+def knuth(g, order, probes: int = 64, seed: int = 1, ipx: bool = True, threads: int = 0):
+    """Knuth random-probe estimates of the level-1 subtrees of `order`'s
+    vertices: (mean maximal cliques, mean tree nodes) per vertex, each an
+    unbiased estimate of that subproblem's exact figure."""
+    o = np.ascontiguousarray(order, dtype=np.uint32)
+    ec = np.zeros(max(o.size, 1), np.float64)
+    en = np.zeros(max(o.size, 1), np.float64)
+    rc = lib.oracle_knuth(g.n, _p(g.row_ptr), _p(g.col_ind), 1 if ipx else 0, _p(o), o.size, probes, seed, threads,
+                          _p(ec), _p(en))
+    if rc != 0:
+        raise RuntimeError(f"oracle_knuth failed with {rc}")
+    return ec[:o.size], en[:o.size]

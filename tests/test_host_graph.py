@@ -123,3 +123,18 @@ def test_from_edges_normalizes(mce):
     g = mce.from_edges([0, 1, 1, 2, 3], [1, 0, 1, 0, 2], n=5)
     assert g.m == 6
     g.validate()
+
+
+def test_config_digests(mce):
+    """The benchmark workloads are pinned: the generators' CSR digests for
+    the Erdos-Renyi, R-MAT scale 20 and full-size Chung-Lu (evaluation seed 9)
+    configurations equal those committed in tests/golden/configs.json (the
+    SBM batch and R-MAT scale 24 are checked where they are generated: the
+    GPU tests and bench.py)."""
+    import json
+    import os
+    cfg = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "configs.json")))
+    for name in ("c1", "c2", "c3_seed9"):
+        fx = cfg[name]
+        g = getattr(mce, fx["generator"])(**fx["args"])
+        assert (g.n, g.m, f"{g.digest():#018x}") == (fx["n"], fx["m"], fx["digest"]), name
