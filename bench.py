@@ -177,6 +177,9 @@ def main():
     ap.add_argument("--cpu-sample", type=float, default=1.0 / 64,
                     help="fraction of member graphs (batch) or first-level vertices timed on the CPU")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--peaks", action="store_true",
+                    help="only measure the device's VALU lane-op/s and LDS byte/s peaks (mce_device_peaks), print "
+                         "them as JSON and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (the reference kernel for the HBM "
                          "traffic counters) and exit")
@@ -189,6 +192,12 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copied_bytes": a.numel() * 8}))
+        return
+    if args.peaks:
+        mce = _import("mce_gpu_amd", os.path.join(ROOT, "mce-gpu_amd", "__init__.py"))
+        runs = [mce.device_peaks() for _ in range(3)]
+        print(json.dumps({"valu_lane_ops_per_s": max(r["valu_lane_ops_per_s"] for r in runs),
+                          "lds_bytes_per_s": max(r["lds_bytes_per_s"] for r in runs), "runs": runs}))
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the engine pins the device count to 1",
@@ -207,8 +216,10 @@ def main():
     desc["digest"] = pinned_digest(args.config, args.seed, args.cl_scale, g, off)
 
     model_path = args.model if args.schedule == "learned" else None
+    # progress_s: an enumeration launch longer than half a minute prints its
+    # own counters (tickets claimed, tasks outstanding) while it runs
     kw = dict(level=args.level, induced=args.induced, worklist=args.worklist,
-              schedule=SCHEDULES[args.schedule], model_path=model_path)
+              schedule=SCHEDULES[args.schedule], model_path=model_path, progress_s=30)
     if off is not None:
         # the batched engine: one persistent enumeration launch captured into a hipGraph
         kw["graph_launch"] = 1
@@ -354,6 +365,29 @@ def main():
                                "kind": "port", "sample": f"{args.config} {what}: {ref['count']} cliques, "
                                                          f"order+orient+count timed together"}
         out["parity_sample"] = {"count": ref["count"], "hash": f"{ref['hash']:#018x}", "bit_exact": ok}
+        # The full roofline t_roof = max(B_hbm / HBM peak, OPS / VALU peak,
+        # B_lds / LDS peak). OPS and B_lds are the algorithmic work of the
+        # deterministic BK tree (per node: 2 32-bit lane-ops for the AND and
+        # the POPCNT of every 64-bit word of every candidate pivot row, 4 words
+        # of set updates per child; 8 bytes of LDS per word read), counted by
+        # the oracle on the same sample (untimed) and scaled by the GPU's tree
+        # nodes; the VALU and LDS peaks are measured on this device.
+        wk = oracle.mce(sg, level=args.level, ipx=args.induced == 2, threads=threads, order=order, work=True)
+        peaks = mce.device_peaks()
+        ops = wk["ops"] / max(wk["tree_nodes"], 1) * first["tree_nodes"]
+        b_lds = wk["b_lds"] / max(wk["tree_nodes"], 1) * first["tree_nodes"]
+        parts = {"hbm": algo_b / (HBM_PEAK_GBS * 1e9), "valu": ops / peaks["valu_lane_ops_per_s"],
+                 "lds": b_lds / peaks["lds_bytes_per_s"]}
+        t_roof = max(parts.values())
+        out["roofline"]["t_roof"] = {
+            "seconds": parts, "binding": max(parts, key=parts.get), "t_roof_s": t_roof,
+            "t_kernel_s": kavg / 1e3, "frac": t_roof / (kavg / 1e3), "ops": ops, "lds_bytes": b_lds,
+            "hbm_bytes": algo_b, "ops_per_node": wk["ops"] / max(wk["tree_nodes"], 1),
+            "lds_bytes_per_node": wk["b_lds"] / max(wk["tree_nodes"], 1),
+            "valu_peak_lane_ops_per_s": peaks["valu_lane_ops_per_s"], "lds_peak_bytes_per_s": peaks["lds_bytes_per_s"],
+            "source": "oracle work counters on the CPU sample, scaled by tree nodes; peaks from mce_device_peaks"}
+        log(f"[bench] roofline: {parts} -> binding {out['roofline']['t_roof']['binding']}, "
+            f"t_roof/t_kernel = {out['roofline']['t_roof']['frac']:.4f}")
     print(json.dumps(out))
 
 

@@ -128,6 +128,11 @@ typedef struct mce_opts {
                               a busy wave) for this long stops and the call
                               fails with MCE_EINTERNAL instead of hanging
                               (0 = 120 s)                                   */
+  int progress_s;          /* > 0: while the enumeration launch runs, print
+                              its own counters to stderr every this many
+                              seconds (first-level tickets claimed; with
+                              the worker list, tasks outstanding and the
+                              busy waves' node heartbeats); 0 = silent      */
 
   int schedule;            /* enum mce_schedule                              */
   const uint32_t *order;   /* MCE_SCHED_GIVEN: first-level vertex order      */
@@ -210,6 +215,13 @@ const char *mce_last_error(void);
 void mce_opts_default(mce_opts *opts);
 /* Number of HIP devices (0 when none). Replaces CUDAContext (utils.cuh:37-60). */
 int mce_device_count(void);
+/* Measured peaks of the enumeration kernel's bounding resources on `device`:
+ * 32-bit integer VALU lane-operations per second (independent AND/XOR/POPCNT
+ * chains on every lane of a full GPU) and LDS read bytes per second
+ * (conflict-free 16-byte reads). They price the roofline
+ * t_roof = max(HBM bytes / HBM peak, OPS / VALU peak, LDS bytes / LDS peak).
+ * No reference counterpart (a measurement helper). */
+int mce_device_peaks(int device, double *valu_lane_ops_per_s, double *lds_bytes_per_s);
 
 /* ---- host graph construction (no GPU needed) -------------------------- */
 /* Reads a .bel file: 24-byte records {dst u64, src u64, 0 u64}, sorted by

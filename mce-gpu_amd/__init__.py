@@ -68,7 +68,7 @@ class Opts(ctypes.Structure):
         ("waves_per_group", c_int), ("groups_per_cu", c_int), ("lds_arena_bytes", c_int),
         ("hbm_arena_bytes", c_int), ("stack_lds_levels", c_int), ("donate_min_p", c_int),
         ("donate_check_nodes", c_int), ("donate_age_us", c_int), ("split_heavy", c_int), ("graph_launch", c_int), ("small_universe", c_int),
-        ("watchdog_us", c_int), ("schedule", c_int), ("order", POINTER(c_uint32)),
+        ("watchdog_us", c_int), ("progress_s", c_int), ("schedule", c_int), ("order", POINTER(c_uint32)),
         ("order_len", c_uint64), ("model_path", c_char_p), ("model", POINTER(Mlp)), ("sub_nodes", POINTER(c_uint64)),
         ("sub_cliques", POINTER(c_uint64)),
     ]
@@ -96,6 +96,7 @@ SIGNATURES = {
     "mce_last_error": (c_char_p, []),
     "mce_opts_default": (None, [POINTER(Opts)]),
     "mce_device_count": (c_int, []),
+    "mce_device_peaks": (c_int, [c_int, POINTER(c_double), POINTER(c_double)]),
     "mce_csr_read_bel": (c_int, [c_char_p, POINTER(Csr)]),
     "mce_csr_write_bel": (c_int, [c_char_p, POINTER(Csr)]),
     "mce_convert_text_to_bel": (c_int, [c_char_p, c_char_p, POINTER(c_uint64), POINTER(c_uint64)]),
@@ -262,6 +263,13 @@ def gen_sbm_batch(num_graphs: int, n_min=64, n_max=4096, blocks=4, p_in=0.3, p_o
 
 def device_count() -> int:
     return int(lib.mce_device_count())
+
+
+def device_peaks(device: int = 0) -> dict:
+    """Measured VALU lane-op/s and LDS byte/s peaks of the device."""
+    v, l = c_double(), c_double()
+    _check(lib.mce_device_peaks(device, byref(v), byref(l)))
+    return {"valu_lane_ops_per_s": v.value, "lds_bytes_per_s": l.value}
 
 
 def default_opts(**kw) -> Opts:

@@ -76,7 +76,8 @@ int check_opts(const mce_opts *o) {
   if (o->waves_per_group < 0 || o->waves_per_group > 12) return set_error(MCE_EINVAL, "waves_per_group must be 0..12");
   if (o->groups_per_cu > 16) return set_error(MCE_EINVAL, "groups_per_cu must be 0..16");
   if (o->groups_per_cu < 0 || o->lds_arena_bytes < 0 || o->hbm_arena_bytes < 0 || o->stack_lds_levels < 0 ||
-      o->donate_min_p < 0 || o->donate_check_nodes < 0 || o->donate_age_us < 0 || o->watchdog_us < 0)
+      o->donate_min_p < 0 || o->donate_check_nodes < 0 || o->donate_age_us < 0 || o->watchdog_us < 0 ||
+      o->progress_s < 0)
     return set_error(MCE_EINVAL, "negative block setting");
   if (o->lds_arena_bytes && (o->lds_arena_bytes < 1024 || o->lds_arena_bytes % 8))
     return set_error(MCE_EINVAL, "lds_arena_bytes must be a multiple of 8 and >= 1024");
@@ -176,6 +177,7 @@ void mce_graph_destroy(mce_graph *h) {
   if (!h) return;
   hipSetDevice(h->device);
   if (h->s) hipStreamSynchronize(h->s);
+  h->ws.release();
   hipFree(h->g.rp);
   hipFree(h->g.ci);
   hipFree(h->g.oc);
@@ -269,27 +271,32 @@ static int count_impl(mce_graph *h, const mce_opts *opts, mce_result *res, bool 
   res->t_count_ms = 0;
   if (res->hist && res->hist_len) memset(res->hist, 0, sizeof(uint64_t) * res->hist_len);
 
+  // This call's work list, histogram, schedule bins (twice: a learned
+  // schedule that fails falls back to the plain one), learned keys and
+  // per-root outputs are carved from the graph's kept call buffer; the rare
+  // split-variant buffers are allocated per call.
+  const bool learned_sched = opts->schedule == MCE_SCHED_LEARNED;
+  const uint64_t n64 = g.n;
+  const size_t call_bytes = 4 * n64 + 8ull * kHistLen + 8ull * (2 * (MCE_MAX_OUT_DEGREE + 2) + 1) +
+                            (learned_sched ? 4 * n64 : 0) + (opts->sub_nodes ? 8 * n64 : 0) +
+                            (opts->sub_cliques ? 8 * n64 : 0) + 8 * 256;
+  void *call_base = h->ws.get(kWsCall, call_bytes);
+  if (!call_base) return set_error(MCE_ENOMEM, "device allocation of %zu bytes failed", call_bytes);
+  Scratch call{(uint8_t *)call_base, call_bytes, 0};
   uint32_t *order = nullptr;
-  uint64_t *hist = nullptr, *sub_nodes = nullptr, *sub_cliques = nullptr;
+  uint64_t *hist = (uint64_t *)call.take(sizeof(uint64_t) * kHistLen), *sub_nodes = nullptr, *sub_cliques = nullptr;
   uint2 *arcs = nullptr;
   uint8_t *split_flag = nullptr;
   unsigned long long *n_split = nullptr;
   auto cleanup = [&]() {
     hipFree(split_flag);
     hipFree(n_split);
-    hipFree(order);
-    hipFree(hist);
-    hipFree(sub_nodes);
-    hipFree(sub_cliques);
     hipFree(arcs);
   };
   uint32_t len = 0;
   bool heavy_tail = false, learned_used = false;
   if (g.n) {
-    if (hipMalloc(&order, sizeof(uint32_t) * g.n) != hipSuccess) {
-      cleanup();
-      return set_error(MCE_ENOMEM, "order allocation failed");
-    }
+    order = (uint32_t *)call.take(sizeof(uint32_t) * n64);
     if (opts->schedule == MCE_SCHED_GIVEN) {
       std::vector<uint32_t> keep;
       keep.reserve(opts->order_len);
@@ -317,13 +324,13 @@ static int count_impl(mce_graph *h, const mce_opts *opts, mce_result *res, bool 
         } else if (opts->model_path) {
           have = mce_mlp_load(opts->model_path, &model) == MCE_OK;
         }
-        uint32_t *keys = nullptr;
-        if (have && hipMalloc(&keys, sizeof(uint32_t) * g.n) == hipSuccess) {
+        uint32_t *keys = (uint32_t *)call.take(sizeof(uint32_t) * n64);
+        if (have && keys) {
           uint32_t bad = 0;
           double wsum = 0, wmax = 0;
-          rc = learned_keys(g, h->info, model, keys, &bad, &wsum, &wmax, s);
+          rc = learned_keys(g, h->info, model, keys, &bad, &wsum, &wmax, s, &h->ws);
           if (rc == MCE_OK && bad == 0) {
-            rc = schedule_by_key(g, keys, 4095, order, &len, s);
+            rc = schedule_by_key(g, keys, 4095, order, &len, s, &call);
             learned = rc == MCE_OK;
             learned_used = learned;
             // a predicted subtree larger than a wave's share of the work (16
@@ -346,14 +353,13 @@ static int count_impl(mce_graph *h, const mce_opts *opts, mce_result *res, bool 
               }
             }
           }
-          hipFree(keys);
         }
         if (!learned) fallback++;
         clear_error();
       }
       if (!learned) {
         // plain: heaviest-first by out-degree
-        rc = schedule_by_key(g, g.dplus, std::max<uint32_t>(h->info.max_out_degree, 1), order, &len, s);
+        rc = schedule_by_key(g, g.dplus, std::max<uint32_t>(h->info.max_out_degree, 1), order, &len, s, &call);
         if (rc) {
           cleanup();
           return rc;
@@ -378,17 +384,13 @@ static int count_impl(mce_graph *h, const mce_opts *opts, mce_result *res, bool 
     items = arcs;
     if (opts->level == MCE_L1) level = 3;
   }
-  if (hipMalloc(&hist, sizeof(uint64_t) * kHistLen) != hipSuccess) {
-    cleanup();
-    return set_error(MCE_ENOMEM, "histogram allocation failed");
-  }
   MCE_HIP(hipMemsetAsync(hist, 0, sizeof(uint64_t) * kHistLen, s));
   if (opts->sub_nodes && g.n) {
-    MCE_HIP(hipMalloc(&sub_nodes, sizeof(uint64_t) * g.n));
+    sub_nodes = (uint64_t *)call.take(sizeof(uint64_t) * n64);
     MCE_HIP(hipMemsetAsync(sub_nodes, 0, sizeof(uint64_t) * g.n, s));
   }
   if (opts->sub_cliques && g.n) {
-    MCE_HIP(hipMalloc(&sub_cliques, sizeof(uint64_t) * g.n));
+    sub_cliques = (uint64_t *)call.take(sizeof(uint64_t) * n64);
     MCE_HIP(hipMemsetAsync(sub_cliques, 0, sizeof(uint64_t) * g.n, s));
   }
   BkLaunch cfg;
@@ -410,6 +412,7 @@ static int count_impl(mce_graph *h, const mce_opts *opts, mce_result *res, bool 
   cfg.small_max = opts->small_universe < 0 ? 0u : opts->small_universe ? (uint32_t)opts->small_universe : 64u;
   cfg.hbm_bytes = (uint64_t)opts->hbm_arena_bytes;
   cfg.watchdog_us = (uint32_t)opts->watchdog_us;
+  cfg.progress_s = (uint32_t)opts->progress_s;
   if (heavy_tail && !opts->donate_age_us) {
     // thresholds set from the cost model: with a predicted heavy tail only
     // long-running tasks donate (a donation copies the donor's bit matrix;
@@ -421,7 +424,7 @@ static int count_impl(mce_graph *h, const mce_opts *opts, mce_result *res, bool 
                           opts->task == MCE_TASK_BD_EVAL;
   rc = bk_run(g, h->info, items, n_items, cfg, hist, kHistLen, sub_nodes, sub_cliques,
               want_waves ? res->wave_stats : nullptr, want_waves ? res->wave_stats_cap : 0, &res->wave_stats_len, s,
-              &out);
+              &out, nullptr, &h->ws);
   if (rc) {
     cleanup();
     return rc;

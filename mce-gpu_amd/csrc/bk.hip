@@ -40,9 +40,11 @@
 // work (tickets + handed-over tasks) ends the kernel exactly; a realtime
 // watchdog turns any termination bug into an error flag instead of a hang.
 #include <hip/hip_runtime.h>
+#include <string.h>
 
 #include <algorithm>
 #include <chrono>
+#include <thread>
 #include <vector>
 
 #include "engine.hpp"
@@ -79,15 +81,14 @@ constexpr uint32_t kCtlProgress = 0, kCtlOutstanding = 16, kCtlQueues = 32, kCtl
 constexpr uint32_t kMboxStride = 16;
 constexpr unsigned long long kNoSlot = ~0ull;
 constexpr uint32_t kSlotHeader = 64;  // bytes
-// Launch shape of the enumeration kernel per variant. The worker-list
-// variants carry the most live state (task slots, donation, the idle loop):
-// at 4 waves per SIMD (128 VGPRs) they spill about 520 bytes per lane, so
-// they run at 2 waves per SIMD (256 VGPRs) in workgroups of up to 8 waves
-// (Chung-Lu 2^18: 356 -> 202 ms). The others keep 4 waves per SIMD in
-// workgroups of up to 12: there occupancy is worth more than the spills
-// (32-graph SBM sample: 171 ms at 4, 196 ms at 3, 271 ms at 2).
-constexpr uint32_t max_wpg_of(bool wl) { return wl ? 8u : 12u; }
-constexpr uint32_t waves_per_simd_of(bool wl) { return wl ? 2u : 4u; }
+// Launch shape of the enumeration kernel: at most 12 waves per workgroup, 4
+// waves per SIMD (128 VGPRs), for every variant. Measured alternatives for the
+// worker-list variants, which spill at 128 VGPRs: 2 waves per SIMD (256
+// VGPRs, no spill) took Chung-Lu 2^18 from 353 to 207 ms but 2^20 from 15.2
+// to 22.2 s, 3 waves per SIMD gave 250 ms and 17.2 s; on heavy tails the
+// number of waves that can take donated work counts for more than their speed.
+constexpr uint32_t max_wpg_of(bool) { return 12u; }
+constexpr uint32_t waves_per_simd_of(bool) { return 4u; }
 enum TaskKind : uint32_t { kTaskList = 0, kTaskBits = 1 };
 // Idle waves watch for global progress (a change of the outstanding-work
 // count or of the busy waves' heartbeat counter); after this many 100 MHz
@@ -143,7 +144,17 @@ struct BkParams {
   uint32_t small_max;   // nodes with P + X + outside rows <= this go to the small-universe solver
   uint32_t small_pmax;  // ... and, with the worker list, |P| <= this
   uint64_t watchdog_ticks;  // idle time without global progress before the kernel gives up
+  // pinned host words the kernel mirrors its progress into while it runs
+  // (word 0: a recent ticket claimed, word 1: heartbeats); nullable
+  unsigned long long *host_progress;
 };
+
+// Posts a progress value to pinned host memory (a system-scope vector store;
+// the host polls it while the persistent launch holds every CU, when a copy
+// from device memory could not be scheduled).
+__device__ __forceinline__ void post_progress(unsigned long long *p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 struct Sub {
   uint32_t *S;
@@ -666,7 +677,10 @@ __device__ __forceinline__ uint32_t queue_len(const BkParams &pr, uint32_t q) {
 __device__ __forceinline__ void heartbeat(const BkParams &pr, WaveAcc &wa) {
   if (wa.nodes - wa.last_beat >= kBeatNodes) {
     wa.last_beat = wa.nodes;
-    if (lane_id() == 0) atomicAdd(pr.ctl + kCtlProgress, 1ull);
+    if (lane_id() == 0) {
+      const unsigned long long b = atomicAdd(pr.ctl + kCtlProgress, 1ull);
+      if (pr.host_progress) post_progress(pr.host_progress + 1, b + 1);
+    }
   }
 }
 
@@ -1791,7 +1805,10 @@ __global__ __launch_bounds__(max_wpg_of(kWL) * kWave, waves_per_simd_of(kWL)) vo
   while (true) {
     clk.to(kBdQueue);
     uint32_t t = 0;
-    if (L == 0) t = atomicAdd(pr.ticket, 1u);
+    if (L == 0) {
+      t = atomicAdd(pr.ticket, 1u);
+      if (pr.host_progress && (t & 1023u) == 0) post_progress(pr.host_progress, t);
+    }
     t = uniform(t);
     uint64_t lo, hi;
     if (t < pr.t1) {
@@ -2191,10 +2208,79 @@ int stat_of(int task) {
 
 }  // namespace
 
+void *Workspace::get(int slot, size_t n) {
+  n = std::max<size_t>(n, 1);
+  if (bytes[slot] >= n) return p[slot];
+  if (p[slot]) {
+    (void)hipFree(p[slot]);
+    p[slot] = nullptr;
+    bytes[slot] = 0;
+  }
+  if (hipMalloc(&p[slot], n) != hipSuccess) {
+    p[slot] = nullptr;
+    return nullptr;
+  }
+  bytes[slot] = n;
+  return p[slot];
+}
+
+size_t Workspace::held() const {
+  size_t b = 0;
+  for (int k : {kWsArena, kWsRing, kWsSeq, kWsIdle, kWsMbox}) b += bytes[k];
+  return b;
+}
+
+void Workspace::drop_graph() {
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  exec = nullptr;
+  graph = nullptr;
+  key.clear();
+}
+
+void Workspace::release() {
+  drop_graph();
+  for (int k = 0; k < kWsSlots; ++k) {
+    if (p[k]) (void)hipFree(p[k]);
+    p[k] = nullptr;
+    bytes[k] = 0;
+  }
+  if (pinned) (void)hipHostFree(pinned);
+  pinned = nullptr;
+}
+
+// Waits for the enumeration launch on `s`. With progress_s > 0 it prints the
+// launch's own counters to stderr every progress_s seconds while it runs:
+// the kernel posts a recent ticket claim and (worker list) its heartbeat
+// count (one per kBeatNodes tree nodes of a long task) into `posted`, pinned
+// host memory.
+static int wait_launch(hipStream_t s, uint32_t progress_s, const volatile unsigned long long *posted,
+                       uint64_t n_tickets, bool wl) {
+  if (progress_s && posted) {
+    const auto t0 = std::chrono::steady_clock::now();
+    auto next = t0 + std::chrono::seconds(progress_s);
+    hipError_t e;
+    while ((e = hipStreamQuery(s)) == hipErrorNotReady) {
+      std::this_thread::sleep_for(std::chrono::milliseconds(20));
+      if (std::chrono::steady_clock::now() < next) continue;
+      next += std::chrono::seconds(progress_s);
+      const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      fprintf(stderr, "[mce] enumeration %.0f s: about %llu of %llu tickets claimed", el,
+              (unsigned long long)std::min<uint64_t>(posted[0], n_tickets), (unsigned long long)n_tickets);
+      if (wl) fprintf(stderr, ", %llu heartbeats", (unsigned long long)posted[1]);
+      fprintf(stderr, "\n");
+      fflush(stderr);
+    }
+    if (e != hipSuccess) return hip_fail(e, "enumeration launch");
+  }
+  MCE_HIP(hipStreamSynchronize(s));
+  return MCE_OK;
+}
+
 int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64_t n_items, const BkLaunch &cfg,
            uint64_t *d_hist, uint32_t hist_len, uint64_t *d_sub_nodes, uint64_t *d_sub_cliques,
            mce_wave_stat *wave_stats, uint32_t wave_stats_cap, uint32_t *wave_stats_len, hipStream_t s, BkOut *out,
-           Scratch *sc) {
+           Scratch *sc, Workspace *ws) {
   *out = BkOut{};
   if (wave_stats_len) *wave_stats_len = 0;
   if (sc && (cfg.wl || wave_stats || cfg.capture))
@@ -2233,13 +2319,19 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   const size_t lds = hist_bytes + kPoolHeader + pool;
   if (lds > lds_cu) return set_error(MCE_EINVAL, "LDS per workgroup %zu exceeds 160 KiB", lds);
 
-  // every device buffer of this call, released on all paths (or carved from
-  // the caller's scratch region)
+  // every device buffer of this call: carved from the caller's scratch
+  // region, kept in the graph's workspace slot `slot` (slot >= 0), or
+  // allocated for this call and released on all paths
   std::vector<void *> bufs;
-  auto dmalloc = [&](void **p, size_t bytes) -> int {
+  auto dmalloc = [&](void **p, size_t bytes, int slot) -> int {
     if (sc) {
       *p = sc->take(std::max<size_t>(bytes, 1));
       if (!*p) return set_error(MCE_ELIMIT, "scratch region too small: %zu more bytes needed", bytes);
+      return MCE_OK;
+    }
+    if (ws && slot >= 0) {
+      *p = ws->get(slot, bytes);
+      if (!*p) return set_error(MCE_ENOMEM, "device allocation of %zu bytes failed", bytes);
       return MCE_OK;
     }
     if (hipMalloc(p, std::max<size_t>(bytes, 1)) != hipSuccess)
@@ -2257,8 +2349,9 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
 
   unsigned long long *acc = nullptr;
   uint32_t *ticket = nullptr;
-  if ((rc = dmalloc((void **)&acc, sizeof(unsigned long long) * (kAccN + 2)))) return rc;
-  if ((rc = dmalloc((void **)&ticket, sizeof(uint32_t)))) return rc;
+  // counters, the 2 need words and the ticket in one buffer
+  if ((rc = dmalloc((void **)&acc, sizeof(unsigned long long) * (kAccN + 3), kWsAcc))) return rc;
+  ticket = (uint32_t *)(acc + kAccN + 2);
   unsigned long long *need = acc + kAccN;
   MCE_HIP(hipMemsetAsync(acc, 0, sizeof(unsigned long long) * (kAccN + 2), s));
   // per-wave HBM arena cap: items that need more are deferred to a second
@@ -2284,8 +2377,12 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   uint64_t arena = std::min<uint64_t>(hneed[0], cap);
   arena = (arena + 255) & ~255ull;
   size_t freeb = 0, totalb = 0;
-  if (sc) freeb = 4 * sc->left();  // the arenas may take all that is left
-  else MCE_HIP(hipMemGetInfo(&freeb, &totalb));
+  if (sc) {
+    freeb = 4 * sc->left();  // the arenas may take all that is left
+  } else {
+    MCE_HIP(hipMemGetInfo(&freeb, &totalb));
+    if (ws) freeb += ws->held();  // kept buffers are reused or regrown
+  }
   if (arena && (uint64_t)groups * wpg * arena > freeb / 4)
     groups = (uint32_t)std::max<uint64_t>(1, (freeb / 4) / ((uint64_t)wpg * arena));
   // Worker list: one task slot per wave. A slot holds a donor's vertex set
@@ -2318,12 +2415,12 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   uint64_t *overflow = nullptr;
   unsigned long long *wave_val = nullptr, *wave_bd = nullptr;
   uint32_t *wave_cu = nullptr;
-  if (arena && (rc = dmalloc((void **)&hbm, arena * waves))) return rc;
-  if (hneed[1] && (rc = dmalloc((void **)&overflow, sizeof(uint64_t) * hneed[1]))) return rc;
+  if (arena && (rc = dmalloc((void **)&hbm, arena * waves, kWsArena))) return rc;
+  if (hneed[1] && (rc = dmalloc((void **)&overflow, sizeof(uint64_t) * hneed[1], kWsOverflow))) return rc;
   if (wave_stats) {
-    if ((rc = dmalloc((void **)&wave_val, sizeof(unsigned long long) * waves))) return rc;
-    if ((rc = dmalloc((void **)&wave_cu, sizeof(uint32_t) * waves))) return rc;
-    if (stat == kStatBd && (rc = dmalloc((void **)&wave_bd, sizeof(unsigned long long) * 7 * waves))) return rc;
+    if ((rc = dmalloc((void **)&wave_val, sizeof(unsigned long long) * waves, -1))) return rc;
+    if ((rc = dmalloc((void **)&wave_cu, sizeof(uint32_t) * waves, -1))) return rc;
+    if (stat == kStatBd && (rc = dmalloc((void **)&wave_bd, sizeof(unsigned long long) * 7 * waves, -1))) return rc;
   }
 
   pr.rp = g.rp;
@@ -2372,49 +2469,109 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
     pr.donate_min_p = std::max<uint32_t>(cfg.donate_min_p, 1);
     pr.donate_check = std::max<uint32_t>(cfg.donate_check_nodes, 1);
     pr.donate_age = (uint64_t)cfg.donate_age_us * 100;
-    if ((rc = dmalloc((void **)&pr.ctl, sizeof(unsigned long long) * kCtlWords))) return rc;
-    if ((rc = dmalloc((void **)&pr.seq, sizeof(unsigned long long) * kQueues * pr.ring_cap))) return rc;
-    if ((rc = dmalloc((void **)&pr.ring, pr.slot_bytes * pr.ring_cap))) return rc;
-    if ((rc = dmalloc((void **)&pr.idle_ids, sizeof(uint32_t) * kQueues * pr.ring_cap))) return rc;
-    if ((rc = dmalloc((void **)&pr.mbox, sizeof(unsigned long long) * kMboxStride * pr.ring_cap))) return rc;
+    if ((rc = dmalloc((void **)&pr.ctl, sizeof(unsigned long long) * kCtlWords, kWsCtl))) return rc;
+    if ((rc = dmalloc((void **)&pr.seq, sizeof(unsigned long long) * kQueues * pr.ring_cap, kWsSeq))) return rc;
+    if ((rc = dmalloc((void **)&pr.ring, pr.slot_bytes * pr.ring_cap, kWsRing))) return rc;
+    if ((rc = dmalloc((void **)&pr.idle_ids, sizeof(uint32_t) * kQueues * pr.ring_cap, kWsIdle))) return rc;
+    if ((rc = dmalloc((void **)&pr.mbox, sizeof(unsigned long long) * kMboxStride * pr.ring_cap, kWsMbox))) return rc;
     if (!cfg.capture) ring_init<<<64, 256, 0, s>>>(pr.seq, pr.ring_cap, pr.ctl, n_tickets, pr.mbox);
   }
 
+  // pinned host words for the kernel's progress posts: the workspace's (kept
+  // with the graph) or this call's own, freed on all paths
+  struct Pinned {
+    unsigned long long *h = nullptr;
+    bool own = false;
+    ~Pinned() {
+      if (own && h) (void)hipHostFree(h);
+    }
+  } pin;
+  volatile unsigned long long *posted = nullptr;
+  if (cfg.progress_s && !sc) {
+    if (ws && ws->pinned) {
+      pin.h = ws->pinned;
+    } else if (hipHostMalloc((void **)&pin.h, 2 * sizeof(unsigned long long), hipHostMallocMapped) == hipSuccess) {
+      if (ws) ws->pinned = pin.h;
+      else pin.own = true;
+    } else {
+      pin.h = nullptr;
+    }
+    void *dp = nullptr;
+    if (pin.h && hipHostGetDevicePointer(&dp, pin.h, 0) == hipSuccess) {
+      pin.h[0] = pin.h[1] = 0;
+      posted = pin.h;
+      pr.host_progress = (unsigned long long *)dp;
+    }
+  }
   // device-time events, except from a scratch region (event creation may
   // allocate): then host time around the synchronised launch
-  hipEvent_t e0 = nullptr, e1 = nullptr;
+  struct Events {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~Events() {
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+    }
+  } ev;
   if (!sc) {
-    MCE_HIP(hipEventCreate(&e0));
-    MCE_HIP(hipEventCreate(&e1));
+    MCE_HIP(hipEventCreate(&ev.a));
+    MCE_HIP(hipEventCreate(&ev.b));
   }
+  hipEvent_t e0 = ev.a, e1 = ev.b;
   const auto host_t0 = std::chrono::steady_clock::now();
   MCE_HIP(hipMemsetAsync(ticket, 0, sizeof(uint32_t), s));
   if (cfg.capture) {
-    // one persistent launch, captured into a graph: counters reset by a kernel
-    // (no memset nodes), then the worker-list set-up and the enumeration kernel
+    // one persistent launch captured into a graph: counters reset by a kernel
+    // (no memset nodes), then the worker-list set-up and the enumeration
+    // kernel. A graph handle keeps the instantiated graph and relaunches it
+    // while every launch parameter (buffers, settings, work list) is the same.
+    std::vector<uint8_t> key(sizeof(BkParams));
+    memcpy(key.data(), &pr, sizeof(BkParams));  // pr is zero-initialised, padding included
+    for (uint64_t v : {(uint64_t)groups, (uint64_t)wpg, (uint64_t)lds, (uint64_t)stat, (uint64_t)cfg.ipx,
+                       (uint64_t)cfg.wl, (uint64_t)n_tickets})
+      key.insert(key.end(), (const uint8_t *)&v, (const uint8_t *)&v + sizeof(v));
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    MCE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    reset_kernel<<<1, 64, 0, s>>>(acc, kAccN, ticket);
-    if (cfg.wl) ring_init<<<64, 256, 0, s>>>(pr.seq, pr.ring_cap, pr.ctl, n_tickets, pr.mbox);
-    rc = dispatch(cfg.ipx, cfg.wl, stat, false, nullptr, pr, groups, wpg, lds, s);
-    const hipError_t ec = hipStreamEndCapture(s, &graph);
-    if (rc) return rc;
-    MCE_HIP(ec);
-    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (ei != hipSuccess) {
-      hipGraphDestroy(graph);
-      return hip_fail(ei, "hipGraphInstantiate");
+    const bool reuse = ws && ws->exec && ws->key == key;
+    if (reuse) {
+      exec = ws->exec;
+    } else {
+      if (ws) ws->drop_graph();
+      MCE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      reset_kernel<<<1, 64, 0, s>>>(acc, kAccN, ticket);
+      if (cfg.wl) ring_init<<<64, 256, 0, s>>>(pr.seq, pr.ring_cap, pr.ctl, n_tickets, pr.mbox);
+      rc = dispatch(cfg.ipx, cfg.wl, stat, false, nullptr, pr, groups, wpg, lds, s);
+      const hipError_t ec = hipStreamEndCapture(s, &graph);
+      if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      MCE_HIP(ec);
+      const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (ei != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return hip_fail(ei, "hipGraphInstantiate");
+      }
+      if (ws) {
+        ws->graph = graph;
+        ws->exec = exec;
+        ws->key = std::move(key);
+      }
     }
     MCE_HIP(hipEventRecord(e0, s));
     const hipError_t el = hipGraphLaunch(exec, s);
-    MCE_HIP(hipStreamSynchronize(s));
-    hipGraphExecDestroy(exec);
-    hipGraphDestroy(graph);
+    const int rw = el == hipSuccess ? wait_launch(s, cfg.progress_s, posted, n_tickets, cfg.wl) : MCE_OK;
+    if (!ws) {
+      (void)hipGraphExecDestroy(exec);
+      (void)hipGraphDestroy(graph);
+    } else if (el != hipSuccess || rw) {
+      ws->drop_graph();
+    }
     MCE_HIP(el);
+    if (rw) return rw;
   } else {
     if (e0) MCE_HIP(hipEventRecord(e0, s));
     if ((rc = dispatch(cfg.ipx, cfg.wl, stat, false, nullptr, pr, groups, wpg, lds, s))) return rc;
+    if ((rc = wait_launch(s, cfg.progress_s, posted, n_tickets, cfg.wl))) return rc;
   }
   // items that did not fit: rerun them with arenas sized for the largest.
   // The deferred count is always read: an item deferred beyond what
@@ -2429,7 +2586,7 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
     const uint32_t g2 = (uint32_t)std::max<uint64_t>(
         1, std::min<uint64_t>((novf + wpg - 1) / wpg, (freeb / 2) / ((uint64_t)wpg * big)));
     uint8_t *hbm2 = nullptr;
-    if ((rc = dmalloc((void **)&hbm2, big * g2 * wpg))) return rc;
+    if ((rc = dmalloc((void **)&hbm2, big * g2 * wpg, -1))) return rc;
     std::vector<uint64_t> idx(novf);
     MCE_HIP(hipMemcpy(idx.data(), overflow, sizeof(uint64_t) * novf, hipMemcpyDeviceToHost));
     std::sort(idx.begin(), idx.end());
@@ -2438,7 +2595,7 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
     for (uint64_t i = 0; i < novf; ++i)
       MCE_HIP(hipMemcpy(hitems.data() + i * isz, (const uint8_t *)d_items + idx[i] * isz, isz, hipMemcpyDeviceToHost));
     void *ditems2 = nullptr;
-    if ((rc = dmalloc(&ditems2, isz * novf))) return rc;
+    if ((rc = dmalloc(&ditems2, isz * novf, -1))) return rc;
     MCE_HIP(hipMemcpy(ditems2, hitems.data(), isz * novf, hipMemcpyHostToDevice));
     BkParams p2 = pr;
     p2.items = ditems2;
@@ -2464,8 +2621,6 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   float ms = 0;
   if (e0) {
     MCE_HIP(hipEventElapsedTime(&ms, e0, e1));
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
   } else {
     ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - host_t0).count();
   }

@@ -51,6 +51,7 @@ void usage() {
           "\n    --small-universe N    nodes with |P|+|X| <= N (1..64) run one lane per member; -1 = never"
           "\n    --hbm-arena BYTES     HBM arena cap per wave (default 16 MiB; larger subproblems run in a second launch)"
           "\n    --watchdog-us US      worker list: give up after this long without progress (default 120 s)"
+          "\n    --progress S          print the enumeration's progress counters to stderr every S seconds"
           "\n    --schedule plain|learned   first-level order (default plain)"
           "\n    --model PATH          cost model for --schedule learned"
           "\n"
@@ -104,6 +105,7 @@ int main(int argc, char **argv) {
                                      {"small-universe", required_argument, 0, 10},
                                      {"hbm-arena", required_argument, 0, 11},
                                      {"watchdog-us", required_argument, 0, 12},
+                                     {"progress", required_argument, 0, 13},
                                      {0, 0, 0, 0}};
   int opt;
   while ((opt = getopt_long(argc, argv, "g:r:d:m:p:i:w:h", longopts, nullptr)) >= 0) {
@@ -165,6 +167,7 @@ int main(int argc, char **argv) {
       case 10: opts.small_universe = atoi(optarg); break;
       case 11: opts.hbm_arena_bytes = atoi(optarg); break;
       case 12: opts.watchdog_us = atoi(optarg); break;
+      case 13: opts.progress_s = atoi(optarg); break;
       default:
         fprintf(stderr, "\nUnrecognized option!\n");
         usage();

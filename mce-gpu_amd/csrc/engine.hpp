@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/mce.h"
 
 namespace mce {
@@ -36,6 +38,40 @@ struct OrderInfo {
 int peel(const DevGraph &g, hipStream_t s, OrderInfo *info);
 // Orientation into g.oc / g.split / g.dplus plus degree maxima.
 int orient(DevGraph &g, hipStream_t s, OrderInfo *info);
+
+// Device buffers a graph handle keeps across counting calls: each slot grows
+// to the largest size asked for and is freed with the handle, so repeated
+// calls on one graph allocate nothing once warm. Only buffers are kept,
+// never results: every call recomputes and clears what it uses. Also kept:
+// the pinned host words of the progress posts, and the captured enumeration
+// launch, relaunched while its launch parameters are byte-identical.
+enum WsSlot : int {
+  kWsCall = 0,  // work list + histogram + schedule bins (carved as a Scratch)
+  kWsFeat,      // learned schedule: features and predictions
+  kWsAcc,       // enumeration counters and ticket
+  kWsArena,     // per-wave HBM arenas
+  kWsOverflow,  // deferred item list
+  kWsCtl,       // worker list: control words, queues, task slots, mailboxes
+  kWsSeq,
+  kWsRing,
+  kWsIdle,
+  kWsMbox,
+  kWsSlots
+};
+struct Workspace {
+  void *p[kWsSlots] = {};
+  size_t bytes[kWsSlots] = {};
+  unsigned long long *pinned = nullptr;  // 2 words of mapped host memory
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<uint8_t> key;  // launch parameters the captured graph holds
+  // The slot's buffer, grown to at least n bytes (nullptr if that fails).
+  void *get(int slot, size_t n);
+  // Device bytes held by the arena and worker-list slots (reusable budget).
+  size_t held() const;
+  void drop_graph();
+  void release();
+};
 
 // A fixed device region handed out front to back, 256-byte aligned, in place
 // of hipMalloc (the allocation-free entry point's scratch). Nothing is freed;
@@ -77,6 +113,7 @@ struct BkLaunch {
   uint32_t donate_age_us = 1000;
   uint32_t small_max = 64;  // P + X + outside rows at which a node goes to the small-universe solver (0 = never)
   uint32_t watchdog_us = 0;  // worker list: idle time without global progress before giving up (0 = 120 s)
+  uint32_t progress_s = 0;   // > 0: print the launch's own progress counters to stderr this often
   bool capture = false;  // run the enumeration launch as a captured hipGraph
 };
 
@@ -104,7 +141,8 @@ int arcs_from_order(const DevGraph &g, const uint32_t *order, uint32_t order_len
 int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64_t n_items,
            const BkLaunch &cfg, uint64_t *d_hist, uint32_t hist_len, uint64_t *d_sub_nodes,
            uint64_t *d_sub_cliques, mce_wave_stat *wave_stats, uint32_t wave_stats_cap,
-           uint32_t *wave_stats_len, hipStream_t s, BkOut *out, Scratch *sc = nullptr);
+           uint32_t *wave_stats_len, hipStream_t s, BkOut *out, Scratch *sc = nullptr,
+           Workspace *ws = nullptr);
 
 int hip_fail(hipError_t e, const char *what);
 
@@ -114,8 +152,9 @@ int predict_device(const mce_mlp &m, const float *F, uint64_t rows, float *out, 
 bool mlp_valid(const mce_mlp *m);
 // Schedule keys from the model's predictions; `bad` counts non-finite ones;
 // work_sum / work_max are the predicted tree sizes (expm1 of the predictions).
+// Features and predictions go to ws's kWsFeat slot when ws is given.
 int learned_keys(const DevGraph &g, const OrderInfo &info, const mce_mlp &m, uint32_t *keys, uint32_t *bad,
-                 double *work_sum, double *work_max, hipStream_t s);
+                 double *work_sum, double *work_max, hipStream_t s, Workspace *ws = nullptr);
 
 // Checks an options block against the ABI's ranges and rules.
 int check_opts(const mce_opts *o);
@@ -129,6 +168,7 @@ struct mce_graph {
   mce::DevGraph g;
   mce::OrderInfo info;
   bool ordered = false;
+  mce::Workspace ws;  // per-call device buffers, kept across calls
 };
 
 #define MCE_HIP(call)                                              \

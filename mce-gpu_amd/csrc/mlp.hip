@@ -409,7 +409,7 @@ int predict_device(const mce_mlp &m, const float *F, uint64_t rows, float *out, 
 bool mlp_valid(const mce_mlp *m) { return m && model_ok(m); }
 
 int learned_keys(const DevGraph &g, const OrderInfo &info, const mce_mlp &m, uint32_t *keys, uint32_t *bad_out,
-                 double *work_sum, double *work_max, hipStream_t s) {
+                 double *work_sum, double *work_max, hipStream_t s, Workspace *ws) {
   *bad_out = 0;
   *work_sum = *work_max = 0;
   if (g.n == 0) return MCE_OK;
@@ -417,10 +417,20 @@ int learned_keys(const DevGraph &g, const OrderInfo &info, const mce_mlp &m, uin
   float *F, *pred, *work;
   unsigned int *bad;
   int rc;
-  if ((rc = b.alloc((void **)&F, sizeof(float) * (uint64_t)g.n * kF)) ||
-      (rc = b.alloc((void **)&pred, sizeof(float) * g.n)) || (rc = b.alloc((void **)&work, 2 * sizeof(float))) ||
-      (rc = b.alloc((void **)&bad, sizeof(unsigned int))))
+  // features | predictions | 2 work sums | bad count, 256-byte aligned parts
+  const size_t fb = (sizeof(float) * (uint64_t)g.n * kF + 255) & ~(size_t)255,
+               pb = (sizeof(float) * (uint64_t)g.n + 255) & ~(size_t)255;
+  uint8_t *base = nullptr;
+  if (ws) {
+    base = (uint8_t *)ws->get(kWsFeat, fb + pb + 256);
+    if (!base) return set_error(MCE_ENOMEM, "device allocation for the cost-model features failed");
+  } else if ((rc = b.alloc((void **)&base, fb + pb + 256))) {
     return rc;
+  }
+  F = (float *)base;
+  pred = (float *)(base + fb);
+  work = (float *)(base + fb + pb);
+  bad = (unsigned int *)(base + fb + pb + 128);
   if ((rc = features_device(g, info, F, s))) return rc;
   if ((rc = predict_device(m, F, g.n, pred, s))) return rc;
   MCE_HIP(hipMemsetAsync(work, 0, 2 * sizeof(float), s));

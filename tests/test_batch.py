@@ -87,16 +87,24 @@ def test_autotune_is_result_invariant(mce, oracle):
 @pytest.mark.gpu
 @pytest.mark.parametrize("wl", [1, 2])
 def test_graph_launch_on_resident_graph(mce, oracle, wl):
-    """The captured (hipGraph) enumeration launch on a resident batch, twice in
-    a row with ordinary launches in between, against the oracle."""
+    """The captured (hipGraph) enumeration launch on a resident batch. The graph
+    handle keeps the instantiated graph and replays it while the launch
+    parameters are unchanged: four replays, with ordinary launches, a launch
+    with other settings (a new capture) and host synchronisations in between,
+    each checked against the oracle."""
     off, g = mce.gen_sbm_batch(24, n_min=64, n_max=500, blocks=4, p_in=0.3, p_out=0.02, seed=5)
     ref = oracle.mce(g, level=1, ipx=True, sub_cliques=True)
+    ref_ip = oracle.mce(g, level=1, ipx=False)
     with mce.DeviceGraph(g) as dg:
         dg.order()
-        for _ in range(2):
+        for k in range(4):
             r = dg.count(level=1, induced=2, worklist=wl, graph_launch=1, sub_cliques=True)
             assert (r["count"], r["hash"], r["hist"], r["tree_nodes"]) == (
-                ref["count"], ref["hash"], ref["hist"], ref["tree_nodes"])
-            assert np.array_equal(r["sub_cliques"], ref["sub_cliques"])
+                ref["count"], ref["hash"], ref["hist"], ref["tree_nodes"]), k
+            assert np.array_equal(r["sub_cliques"], ref["sub_cliques"]), k
             r = dg.count(level=1, induced=2, worklist=wl)
             assert (r["count"], r["hash"]) == (ref["count"], ref["hash"])
+            if k == 1:
+                r = dg.count(level=1, induced=1, worklist=wl, graph_launch=1)
+                assert (r["count"], r["hash"], r["tree_nodes"]) == (ref_ip["count"], ref_ip["hash"],
+                                                                    ref_ip["tree_nodes"])

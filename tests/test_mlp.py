@@ -117,6 +117,63 @@ def test_training_reduces_loss(mce):
     assert bytes(m2) == bytes(m)
 
 
+def np_adam(p, grads, lr, wd, bias_mask, b1=0.9, b2=0.999, eps=1e-8):
+    """Adam (Kingma and Ba, 2015) with bias correction and L2 weight decay
+    added to the gradient of the weights (not the biases), in FP64."""
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    for t, g in enumerate(grads, 1):
+        g = g + np.where(bias_mask, 0.0, wd * p)
+        m = b1 * m + (1 - b1) * g
+        v = b2 * v + (1 - b2) * g * g
+        p = p - lr * (m / (1 - b1 ** t)) / (np.sqrt(v / (1 - b2 ** t)) + eps)
+    return p
+
+
+def flat(P):
+    return np.concatenate([P["W1"].ravel(), P["b1"], P["W2"].ravel(), P["b2"], P["w3"], [P["b3"]]]).astype(np.float64)
+
+
+@pytest.mark.gpu
+def test_adam_steps_match_numpy(mce):
+    """One and two full-batch optimiser steps of the GPU trainer against an
+    FP64 numpy Adam on the FP64 gradient: after one step every weight moves by
+    lr * g / (|g| + eps), so the directions must agree wherever the gradient
+    is not vanishingly small; after two steps the bias-corrected moments set
+    the size of the second move."""
+    x, y = data(4096, 11)
+    lr, wd = 1e-3, 1e-2
+    m0 = mce.mlp_init(x, seed=5)
+    P0 = mce.mlp_params(m0)
+    p0 = flat(P0)
+    sizes = [16 * 64, 64, 64 * 64, 64, 64, 1]
+    bias = np.concatenate([np.full(k, b) for k, b in zip(sizes, (False, True, False, True, False, True))])
+    g1, _ = np_grad(P0, x, y.astype(np.float64))
+    m1 = mce.mlp_init(x, seed=5)
+    mce.mlp_train(m1, x, y, epochs=1, batch=x.shape[0], lr=lr, weight_decay=wd, seed=1)
+    p1 = flat(mce.mlp_params(m1))
+    want1 = np_adam(p0, [g1], lr, wd, bias)
+    gw = np.abs(g1 + np.where(bias, 0.0, wd * p0))
+    ok = gw > 1e-4
+    assert ok.mean() > 0.9
+    np.testing.assert_allclose(p1[ok], want1[ok], rtol=0, atol=1e-6)
+    assert np.all(np.abs(p1 - p0) <= lr * (1 + 1e-5))
+    # second step from the numpy-updated weights
+    P1 = dict(P0)
+    o = np.cumsum([0] + sizes)
+    w = want1
+    P1.update(W1=w[o[0]:o[1]].reshape(16, 64), b1=w[o[1]:o[2]], W2=w[o[2]:o[3]].reshape(64, 64), b2=w[o[3]:o[4]],
+              w3=w[o[4]:o[5]], b3=float(w[o[5]]))
+    g2, _ = np_grad(P1, x, y.astype(np.float64))
+    m2 = mce.mlp_init(x, seed=5)
+    mce.mlp_train(m2, x, y, epochs=2, batch=x.shape[0], lr=lr, weight_decay=wd, seed=1)
+    p2 = flat(mce.mlp_params(m2))
+    want2 = np_adam(p0, [g1, g2], lr, wd, bias)
+    g2w = np.abs(g2 + np.where(bias, 0.0, wd * want1))
+    ok2 = ok & (g2w > 1e-4)
+    np.testing.assert_allclose(p2[ok2], want2[ok2], rtol=0, atol=1e-5)
+
+
 @pytest.mark.gpu
 def test_learned_schedule_is_result_invariant(mce, oracle, tmp_path):
     g = mce.gen_rmat(12, 16, seed=13)

@@ -222,15 +222,26 @@ def test_peel_rounds_are_batches(mce, oracle):
     deg = np.diff(g.row_ptr).astype(np.int64)
     first = np.nonzero(prio == 0)[0]
     assert set(first.tolist()) == set(np.nonzero(deg == deg.min())[0].tolist())
+    # a round belongs to one level of the peel, and that level is the core
+    # number of every vertex it removes (Batagelj-Zaversnik, computed
+    # independently): each vertex leaves with at most that many neighbours
+    core, k = oracle.core_numbers(g)
+    assert k == d
     removed = np.zeros(g.n, bool)
+    levels = []
     for r in range(rounds):
         batch = np.nonzero(prio == r)[0]
         if batch.size == 0:
             continue
+        level = set(core[batch].tolist())
+        assert len(level) == 1, (r, level)
+        level = level.pop()
         rem = [int(np.sum(~removed[g.col_ind[g.row_ptr[v]:g.row_ptr[v + 1]]])) for v in batch]
-        assert max(rem) <= d
+        assert max(rem) <= level, (r, level, max(rem))
+        levels.append(level)
         removed[batch] = True
     assert removed.all()
+    assert levels == sorted(levels) and levels[-1] == d
 
 
 def test_schedule_invariance_and_labels(mce, oracle):
@@ -242,3 +253,34 @@ def test_schedule_invariance_and_labels(mce, oracle):
     r = oracle.mce(g, level=1, ipx=True, order=perm, threads=3)
     assert (r["count"], r["hash"], r["hist"], r["tree_nodes"]) == (
         base["count"], base["hash"], base["hist"], base["tree_nodes"])
+
+
+def test_knuth_estimator(mce, oracle):
+    """Knuth's random-probe estimate of a subtree (used to size the R-MAT
+    workloads) is unbiased: where the pivot leaves one branch per node it is
+    exact, and over many probes it converges to the exact per-vertex clique
+    and node counts of the enumeration."""
+    # K_8: vertex 0's subproblem is one path down to the single clique
+    g = graph(mce, 8, list(itertools.combinations(range(8), 2)))
+    ec, en = oracle.knuth(g, np.arange(8, dtype=np.uint32), probes=16)
+    assert ec.sum() == 1.0
+    # Moon-Moser t = 4: 81 cliques, every vertex subproblem sampled
+    t = 4
+    tri = {(3 * i + a, 3 * i + b) for i in range(t) for a, b in ((0, 1), (0, 2), (1, 2))}
+    g = graph(mce, 3 * t, [e for e in itertools.combinations(range(3 * t), 2) if e not in tri])
+    ec, en = oracle.knuth(g, np.arange(3 * t, dtype=np.uint32), probes=20000, seed=3)
+    ref = oracle.mce(g, level=1, ipx=True, sub_nodes=True, sub_cliques=True)
+    assert abs(ec.sum() - 81) < 0.03 * 81
+    # the heaviest R-MAT subtrees, vertex by vertex
+    g = mce.gen_rmat(12, 16, seed=3)
+    ref = oracle.mce(g, level=1, ipx=True, sub_nodes=True, sub_cliques=True)
+    top = np.argsort(-ref["sub_nodes"].astype(np.float64))[:4].astype(np.uint32)
+    for ipx in (True, False):
+        ec, en = oracle.knuth(g, top, probes=50000, seed=5, ipx=ipx)
+        want = oracle.mce(g, level=1, ipx=ipx, sub_nodes=True, sub_cliques=True)
+        assert np.all(np.abs(ec - want["sub_cliques"][top]) < 0.05 * want["sub_cliques"][top]), (ipx, ec)
+        assert np.all(np.abs(en - want["sub_nodes"][top]) < 0.05 * want["sub_nodes"][top]), (ipx, en)
+    # all vertices of the Erdos-Renyi configuration: the total
+    g = mce.gen_er(4096, 65536, 1)
+    ec, en = oracle.knuth(g, np.arange(g.n, dtype=np.uint32), probes=500, seed=7)
+    assert abs(ec.sum() - 56449) < 0.01 * 56449
