@@ -15,9 +15,11 @@ Workloads (--config), all seeded:
   c4  4,096 stochastic-block-model graphs, n ~ U[64,4096], 4 blocks, p_in .3, p_out .01
   c5  R-MAT scale 24, edge factor 32
 The default is c4, the largest configuration whose exact enumeration finishes
-on one GPU in about a minute; c2 and c5 hold on the order of 1e16 maximal
-cliques (the R-MAT clique count grows 10-25x per scale: 0.65M, 6.3M, 129M,
-3.3B at scales 12-15) and cannot be enumerated exactly by any engine.
+on one GPU in about a minute. By Knuth's tree-size estimator over the oracle's
+BK tree, c2 holds about 4e12 maximal cliques (1e13 tree nodes), the full c3
+about 3e12 (9.5e12 nodes) and c5 about 5e21: hours of enumeration for c2 and
+c3, out of reach of any engine for c5. c3 runs at smaller scales with
+--cl-scale (2^21 takes about ten minutes).
 
 Prints one JSON line (rank 0, N=1).
 """
@@ -177,6 +179,11 @@ def main():
     ap.add_argument("--cpu-sample", type=float, default=1.0 / 64,
                     help="fraction of member graphs (batch) or first-level vertices timed on the CPU")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--write-bel", default=None, metavar="PATH",
+                    help="only write the selected workload (--config, --seed; c1, c2, c3, c5) as a .bel file for "
+                         "the parallel_mce_on_gpus command line and exit")
+    ap.add_argument("--rmat-scale", type=int, default=0,
+                    help="with --write-bel: an R-MAT graph of this scale (edge factor 16, seed --seed) instead")
     ap.add_argument("--peaks", action="store_true",
                     help="only measure the device's VALU lane-op/s and LDS byte/s peaks (mce_device_peaks), print "
                          "them as JSON and exit")
@@ -192,6 +199,17 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copied_bytes": a.numel() * 8}))
+        return
+    if args.write_bel:
+        mce = _import("mce_gpu_amd", os.path.join(ROOT, "mce-gpu_amd", "__init__.py"))
+        if args.rmat_scale:
+            g = mce.gen_rmat(args.rmat_scale, 16, seed=args.seed)
+        else:
+            g, off, _ = make_workload(mce, args.config, args.seed, args.cl_scale)
+            if off is not None:
+                raise SystemExit("a batch has no single-graph .bel form")
+        g.write_bel(args.write_bel)
+        print(json.dumps({"bel": args.write_bel, "n": g.n, "arcs": g.m, "digest": f"{g.digest():#018x}"}))
         return
     if args.peaks:
         mce = _import("mce_gpu_amd", os.path.join(ROOT, "mce-gpu_amd", "__init__.py"))

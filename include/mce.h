@@ -325,7 +325,10 @@ int mce_run_batch(const mce_batch *batch, const mce_opts *opts,
 
 /* Autotuner: times candidate block settings (waves per workgroup and
  * workgroups per CU, which set how many waves share one LDS pool; the BK
- * frame levels a subproblem asks the pool for; donation thresholds) on a
+ * frame levels a subproblem asks the pool for; the small-universe threshold;
+ * the LDS share per wave, which decides which bit matrices stay in LDS;
+ * donation thresholds, which set how many waves a heavy subproblem spreads
+ * over) on a
  * stride sample of the first-level subproblems of an ordered graph (at least
  * 64K of them, or all), heaviest first, and writes the fastest into *best;
  * report (nullable) receives a JSON list of every trial. With base worklist

@@ -382,7 +382,7 @@ class DeviceGraph:
         buf = ctypes.create_string_buffer(1 << 16)
         _check(lib.mce_autotune(self.handle, byref(base), sample_fraction, byref(best), buf, len(buf)))
         keys = ("worklist", "waves_per_group", "groups_per_cu", "stack_lds_levels", "donate_min_p",
-                "donate_check_nodes", "donate_age_us")
+                "donate_check_nodes", "donate_age_us", "small_universe", "lds_arena_bytes")
         return {k: getattr(best, k) for k in keys}, _json.loads(buf.value.decode() or "[]")
 
     def close(self) -> None:

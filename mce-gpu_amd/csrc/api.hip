@@ -567,7 +567,7 @@ int mce_autotune(mce_graph *h, const mce_opts *base, double sample_fraction, mce
     return rp[x + 1] - split[x] > rp[y + 1] - split[y];
   });
   struct Cand {
-    int wl, wpg, gpc, stack, minp, check, age;
+    int wl, wpg, gpc, stack, minp, check, age, su, lds;
   };
   // the variant (both when the caller leaves it to the tuner), then waves per
   // workgroup x workgroups per CU (each workgroup shares one LDS pool)
@@ -581,11 +581,11 @@ int mce_autotune(mce_graph *h, const mce_opts *base, double sample_fraction, mce
   cur.order = sample.data();
   cur.order_len = sample.size();
   cur.sub_nodes = cur.sub_cliques = nullptr;
-  cur.lds_arena_bytes = 0;
+  cur.lds_arena_bytes = base->lds_arena_bytes;
   mce_result r{};
   double best_ms = 1e300;
   Cand bestc{variants[0], base->waves_per_group, base->groups_per_cu, base->stack_lds_levels, base->donate_min_p,
-             base->donate_check_nodes, base->donate_age_us};
+             base->donate_check_nodes, base->donate_age_us, base->small_universe, base->lds_arena_bytes};
   uint64_t ref_count = 0, ref_hash = 0;
   bool have_ref = false;
   auto run = [&](const Cand &cd) -> int {
@@ -597,6 +597,8 @@ int mce_autotune(mce_graph *h, const mce_opts *base, double sample_fraction, mce
     if (cd.minp) o.donate_min_p = cd.minp;
     if (cd.check) o.donate_check_nodes = cd.check;
     if (cd.age) o.donate_age_us = cd.age;
+    o.small_universe = cd.su;
+    o.lds_arena_bytes = cd.lds;
     memset(&r, 0, sizeof(r));
     const int e = count_impl(h, &o, &r, false);
     if (e) return e;
@@ -607,12 +609,12 @@ int mce_autotune(mce_graph *h, const mce_opts *base, double sample_fraction, mce
     } else if (r.count != ref_count || r.hash != ref_hash) {
       return set_error(MCE_EINTERNAL, "results changed with the block settings");
     }
-    char buf[256];
+    char buf[320];
     snprintf(buf, sizeof(buf), "%s{\"worklist\":%d,\"waves_per_group\":%d,\"groups_per_cu\":%d,"
              "\"stack_lds_levels\":%d,\"donate_min_p\":%d,\"donate_check_nodes\":%d,\"donate_age_us\":%d,"
-             "\"waves\":%u,\"ms\":%.3f}",
-             rep.size() > 1 ? "," : "", cd.wl, cd.wpg, cd.gpc, cd.stack, cd.minp, cd.check, cd.age, r.num_waves,
-             r.t_count_ms);
+             "\"small_universe\":%d,\"lds_arena_bytes\":%d,\"waves\":%u,\"ms\":%.3f}",
+             rep.size() > 1 ? "," : "", cd.wl, cd.wpg, cd.gpc, cd.stack, cd.minp, cd.check, cd.age, cd.su, cd.lds,
+             r.num_waves, r.t_count_ms);
     rep += buf;
     if (r.t_count_ms < best_ms) {
       best_ms = r.t_count_ms;
@@ -622,15 +624,30 @@ int mce_autotune(mce_graph *h, const mce_opts *base, double sample_fraction, mce
   };
   for (int v : variants)
     for (const auto &sh : shapes) {
-      Cand cd{v, sh[0], sh[1], 0, 0, 0, 0};
+      Cand cd{v, sh[0], sh[1], 0, 0, 0, 0, base->small_universe, 0};
       if ((rc = run(cd))) return rc;
     }
-  // then the frame levels a subproblem asks the pool for, then (worker list)
-  // the donation thresholds
+  // then, one knob at a time from the best so far: the frame levels a
+  // subproblem asks the pool for; the small-universe threshold (which
+  // subtrees run one lane per member); the LDS share per wave (which bit
+  // matrices stay in LDS rather than HBM); then (worker list) the donation
+  // thresholds, which set how many waves a heavy subproblem spreads over
   const Cand shape = bestc;
   for (int st : {8, 48}) {
     Cand cd = shape;
     cd.stack = st;
+    if ((rc = run(cd))) return rc;
+  }
+  const Cand framed = bestc;
+  for (int su : {32, 48}) {
+    Cand cd = framed;
+    cd.su = su;
+    if ((rc = run(cd))) return rc;
+  }
+  const Cand small = bestc;
+  for (int lds : {4096, 16384}) {
+    Cand cd = small;
+    cd.lds = lds;
     if ((rc = run(cd))) return rc;
   }
   if (bestc.wl == MCE_WL) {
@@ -646,7 +663,8 @@ int mce_autotune(mce_graph *h, const mce_opts *base, double sample_fraction, mce
   rep += "]";
   *best = *base;
   best->worklist = bestc.wl;
-  best->lds_arena_bytes = 0;
+  best->lds_arena_bytes = bestc.lds;
+  best->small_universe = bestc.su;
   best->waves_per_group = bestc.wpg;
   best->groups_per_cu = bestc.gpc;
   best->stack_lds_levels = bestc.stack;
