@@ -1356,6 +1356,152 @@ __device__ __forceinline__ bool small_ok(const BkParams &pr, uint64_t P, uint64_
   return wave_sum(np + popc64(X) + popc64(E)) <= pr.small_max;
 }
 
+// Mask width of the small-universe solver's loop: a universe of at most 32
+// members runs with 32-bit masks and rows (one register and one lane read per
+// row, one AND and one POPCNT per pivot key, two frame fields fewer), a
+// larger one with 64-bit masks. Members, pivots and child order are the same
+// either way.
+template <typename Mk>
+struct MaskOps;
+template <>
+struct MaskOps<uint32_t> {
+  __device__ static uint32_t popc(uint32_t x) { return (uint32_t)__builtin_popcount(x); }
+  __device__ static uint32_t ctz(uint32_t x) { return (uint32_t)__builtin_ctz(x); }
+  __device__ static uint32_t rd(uint32_t v, uint32_t l) { return rdlane(v, l); }
+  // lanes 32..63 hold no member
+  __device__ static bool bit(uint32_t m, uint32_t L) { return L < 32 && ((m >> (L & 31)) & 1u); }
+};
+template <>
+struct MaskOps<uint64_t> {
+  __device__ static uint32_t popc(uint64_t x) { return popc64(x); }
+  __device__ static uint32_t ctz(uint64_t x) { return ctz64(x); }
+  __device__ static uint64_t rd(uint64_t v, uint32_t l) { return rdlane64(v, l); }
+  __device__ static bool bit(uint64_t m, uint32_t L) { return (m >> L) & 1ull; }
+};
+
+// The small-universe solver's loop over a renumbered universe: lane i holds
+// member i's row M over the members and its hash term hs; lP / lX are the
+// root's P and X over the members.
+template <typename Mk, int kStat>
+__device__ __forceinline__ void small_loop(const BkParams &pr, Mk M, uint64_t hs, Mk lP, Mk lX, uint64_t mix,
+                                           uint32_t rbase, unsigned long long *hist_lds, WaveAcc &wa,
+                                           Clock<kStat> &clk) {
+  using O = MaskOps<Mk>;
+  const uint32_t L = lane_id();
+  // pivot over P | X (the lowest member among the best scores): false when
+  // it covers P (no child)
+  auto pick = [&](Mk P, Mk X, uint32_t &pv) -> bool {
+    const uint32_t key = O::bit(P | X, L) ? O::popc(M & P) + 1u : 0u;
+    const uint32_t mx = wave_max(key);
+    pv = ctz64(ballot(key == mx));
+    return mx != 0 && mx - 1u != O::popc(P);
+  };
+  clk.to(kBdPivot);
+  Mk P = lP, X = lX;
+  uint32_t piv;
+  if (!pick(P, X, piv)) return;
+  Mk Np = O::rd(M, piv);
+  uint32_t d = 0;
+  // frames: lane d of each register holds level d
+  Mk fP = 0, fX = 0;
+  uint32_t fV = 0, fMl = 0, fMh = 0;
+  // tallies of this subtree, added to the wave's totals at the end; the
+  // cliques' hash terms and histogram bins are taken 64 at a time
+  WaveAcc t{};
+  uint64_t &nodes = t.nodes;
+  CliqueBuf kb;
+  while (true) {
+    // an iteration keeps at most two cliques
+    if (kb.n > kWave - 2) flush_cliques(pr, hist_lds, kb);
+    clk.to(kBdSetOps);
+    const Mk B = P & ~Np;
+    if (B == 0) {
+      if (d == 0) break;
+      clk.to(kBdBranch);
+      --d;
+      P = O::rd(fP, d);
+      X = O::rd(fX, d);
+      mix = ((uint64_t)rdlane(fMh, d) << 32) | rdlane(fMl, d);
+      piv = rdlane(fV, d);
+      Np = O::rd(M, piv);
+      continue;
+    }
+    const Mk cb = B & (~B + 1);
+    P ^= cb;
+    X |= cb;
+    const uint32_t c = O::ctz(cb);
+    const Mk Mc = O::rd(M, c);
+    const Mk Pn = P & Mc, Xn = X & Mc;
+    ++nodes;
+    const uint64_t cmix = mix + rdlane64(hs, c);
+    if (Pn == 0) {
+      clk.to(kBdMaximal);
+      if (Xn == 0) keep_clique(pr, hist_lds, t, kb, rbase + d + 1, cmix);
+      continue;
+    }
+    const uint32_t np = O::popc(Pn);
+    if (np <= 2) {
+      // |P| <= 2 decided without a pivot scan, node for node as the scan
+      // would go (see the word-per-lane solver)
+      clk.to(kBdMaximal);
+      const uint32_t a = O::ctz(Pn);
+      const Mk Ma = O::rd(M, a);
+      const uint64_t ha = rdlane64(hs, a);
+      if (np == 1) {
+        if ((Xn & Ma) == 0) {
+          ++nodes;
+          keep_clique(pr, hist_lds, t, kb, rbase + d + 2, cmix + ha);
+        }
+        continue;
+      }
+      const uint32_t b = O::ctz(Pn & (Pn - 1));
+      const Mk Mb = O::rd(M, b);
+      const uint64_t hb = rdlane64(hs, b);
+      if (Xn & Ma & Mb) continue;  // an X member covers P
+      if ((Ma >> b) & 1u) {
+        nodes += 2;
+        keep_clique(pr, hist_lds, t, kb, rbase + d + 3, cmix + ha + hb);
+        continue;
+      }
+      const Mk xs = Xn & (Ma | Mb);
+      if (xs == 0) {
+        nodes += 2;
+        keep_clique(pr, hist_lds, t, kb, rbase + d + 2, cmix + ha);
+        keep_clique(pr, hist_lds, t, kb, rbase + d + 2, cmix + hb);
+        continue;
+      }
+      // the pivot is the lowest X member adjacent to one of a, b; the child
+      // is the other one
+      const bool piv_a = (Ma >> O::ctz(xs)) & 1u;
+      ++nodes;
+      if ((Xn & (piv_a ? Mb : Ma)) == 0) keep_clique(pr, hist_lds, t, kb, rbase + d + 2, cmix + (piv_a ? hb : ha));
+      continue;
+    }
+    clk.to(kBdPivot);
+    uint32_t npiv;
+    if (!pick(Pn, Xn, npiv)) continue;
+    clk.to(kBdBranch);
+    const bool here = L == d;
+    fP = here ? P : fP;
+    fX = here ? X : fX;
+    fMl = here ? (uint32_t)mix : fMl;
+    fMh = here ? (uint32_t)(mix >> 32) : fMh;
+    fV = here ? piv : fV;
+    ++d;
+    P = Pn;
+    X = Xn;
+    mix = cmix;
+    piv = npiv;
+    Np = O::rd(M, piv);
+  }
+  if (kb.n) flush_cliques(pr, hist_lds, kb);
+  wa.nodes += t.nodes;
+  wa.count += t.count;
+  wa.sub_cliques += t.count;
+  wa.hash += wave_add64(kb.hacc);
+  wa.max_size = max(wa.max_size, t.max_size);
+}
+
 template <bool kWL, int kStat, bool kLA, bool kLX>
 __device__ __forceinline__ void solve_small(const BkParams &pr, const Sub &s, uint64_t Pw, uint64_t Xw, uint64_t Ew,
                                             uint64_t mix, uint32_t rbase, unsigned long long *hist_lds, WaveAcc &wa,
@@ -1418,117 +1564,10 @@ __device__ __forceinline__ void solve_small(const BkParams &pr, const Sub &s, ui
   }
   const uint64_t hs = valid && !out ? ld64<kLA>(s.H, g) : 0ull;
 
-  // pivot over P | X (the lowest member among the best scores): false when
-  // it covers P (no child)
-  auto pick = [&](uint64_t P, uint64_t X, uint32_t &pv) -> bool {
-    const uint32_t key = ((P | X) >> L) & 1ull ? popc64(M & P) + 1u : 0u;
-    const uint32_t mx = wave_max(key);
-    pv = ctz64(ballot(key == mx));
-    return mx != 0 && mx - 1u != popc64(P);
-  };
-  clk.to(kBdPivot);
-  uint64_t P = lP, X = lX;
-  uint32_t piv;
-  if (!pick(P, X, piv)) return;
-  uint64_t Np = rdlane64(M, piv);
-  uint32_t d = 0;
-  // frames: lane d of each register holds level d
-  uint32_t fPl = 0, fPh = 0, fXl = 0, fXh = 0, fV = 0, fMl = 0, fMh = 0;
-  // tallies of this subtree, added to the wave's totals at the end; the
-  // cliques' hash terms and histogram bins are taken 64 at a time
-  WaveAcc t{};
-  uint64_t &nodes = t.nodes;
-  CliqueBuf kb;
-  while (true) {
-    // an iteration keeps at most two cliques
-    if (kb.n > kWave - 2) flush_cliques(pr, hist_lds, kb);
-    clk.to(kBdSetOps);
-    const uint64_t B = P & ~Np;
-    if (B == 0) {
-      if (d == 0) break;
-      clk.to(kBdBranch);
-      --d;
-      P = ((uint64_t)rdlane(fPh, d) << 32) | rdlane(fPl, d);
-      X = ((uint64_t)rdlane(fXh, d) << 32) | rdlane(fXl, d);
-      mix = ((uint64_t)rdlane(fMh, d) << 32) | rdlane(fMl, d);
-      piv = rdlane(fV, d);
-      Np = rdlane64(M, piv);
-      continue;
-    }
-    const uint64_t cb = B & (~B + 1);
-    P ^= cb;
-    X |= cb;
-    const uint32_t c = ctz64(cb);
-    const uint64_t Mc = rdlane64(M, c);
-    const uint64_t Pn = P & Mc, Xn = X & Mc;
-    ++nodes;
-    const uint64_t cmix = mix + rdlane64(hs, c);
-    if (Pn == 0) {
-      clk.to(kBdMaximal);
-      if (Xn == 0) keep_clique(pr, hist_lds, t, kb, rbase + d + 1, cmix);
-      continue;
-    }
-    const uint32_t np = popc64(Pn);
-    if (np <= 2) {
-      // |P| <= 2 decided without a pivot scan, node for node as the scan
-      // would go (see the word-per-lane solver)
-      clk.to(kBdMaximal);
-      const uint32_t a = ctz64(Pn);
-      const uint64_t Ma = rdlane64(M, a), ha = rdlane64(hs, a);
-      if (np == 1) {
-        if ((Xn & Ma) == 0) {
-          ++nodes;
-          keep_clique(pr, hist_lds, t, kb, rbase + d + 2, cmix + ha);
-        }
-        continue;
-      }
-      const uint32_t b = ctz64(Pn & (Pn - 1));
-      const uint64_t Mb = rdlane64(M, b), hb = rdlane64(hs, b);
-      if (Xn & Ma & Mb) continue;  // an X member covers P
-      if ((Ma >> b) & 1ull) {
-        nodes += 2;
-        keep_clique(pr, hist_lds, t, kb, rbase + d + 3, cmix + ha + hb);
-        continue;
-      }
-      const uint64_t xs = Xn & (Ma | Mb);
-      if (xs == 0) {
-        nodes += 2;
-        keep_clique(pr, hist_lds, t, kb, rbase + d + 2, cmix + ha);
-        keep_clique(pr, hist_lds, t, kb, rbase + d + 2, cmix + hb);
-        continue;
-      }
-      // the pivot is the lowest X member adjacent to one of a, b; the child
-      // is the other one
-      const bool piv_a = (Ma >> ctz64(xs)) & 1ull;
-      ++nodes;
-      if ((Xn & (piv_a ? Mb : Ma)) == 0) keep_clique(pr, hist_lds, t, kb, rbase + d + 2, cmix + (piv_a ? hb : ha));
-      continue;
-    }
-    clk.to(kBdPivot);
-    uint32_t npiv;
-    if (!pick(Pn, Xn, npiv)) continue;
-    clk.to(kBdBranch);
-    const bool here = L == d;
-    fPl = here ? (uint32_t)P : fPl;
-    fPh = here ? (uint32_t)(P >> 32) : fPh;
-    fXl = here ? (uint32_t)X : fXl;
-    fXh = here ? (uint32_t)(X >> 32) : fXh;
-    fMl = here ? (uint32_t)mix : fMl;
-    fMh = here ? (uint32_t)(mix >> 32) : fMh;
-    fV = here ? piv : fV;
-    ++d;
-    P = Pn;
-    X = Xn;
-    mix = cmix;
-    piv = npiv;
-    Np = rdlane64(M, piv);
-  }
-  if (kb.n) flush_cliques(pr, hist_lds, kb);
-  wa.nodes += t.nodes;
-  wa.count += t.count;
-  wa.sub_cliques += t.count;
-  wa.hash += wave_add64(kb.hacc);
-  wa.max_size = max(wa.max_size, t.max_size);
+  if (nm <= 32)
+    small_loop<uint32_t, kStat>(pr, (uint32_t)M, hs, (uint32_t)lP, (uint32_t)lX, mix, rbase, hist_lds, wa, clk);
+  else
+    small_loop<uint64_t, kStat>(pr, M, hs, lP, lX, mix, rbase, hist_lds, wa, clk);
 }
 
 // Runs BK from (P, X, E) on an arena that build_bits (or a received task)
