@@ -184,6 +184,11 @@ def main():
                          "the parallel_mce_on_gpus command line and exit")
     ap.add_argument("--rmat-scale", type=int, default=0,
                     help="with --write-bel: an R-MAT graph of this scale (edge factor 16, seed --seed) instead")
+    ap.add_argument("--rank-sample", type=int, default=0, metavar="N",
+                    help="only score the cost model's ranking: exact tree sizes of N first-level subproblems drawn "
+                         "uniformly (seeded) from the workload, by one given-order enumeration, and the Spearman rho "
+                         "of the model's predictions and of out-degree against them; prints JSON and exits (for "
+                         "graphs too large to label whole)")
     ap.add_argument("--peaks", action="store_true",
                     help="only measure the device's VALU lane-op/s and LDS byte/s peaks (mce_device_peaks), print "
                          "them as JSON and exit")
@@ -227,6 +232,24 @@ def main():
         raise SystemExit("no GPU visible to libmce.so")
     t0 = time.time()
     g, off, desc = make_workload(mce, args.config, args.seed, args.cl_scale)
+    if args.rank_sample:
+        fam = {"c3": "cl", "c4": "sbm"}.get(args.config, "sbm")
+        model = args.model or os.path.join(ROOT, "mce-gpu_amd", f"cost_model_{fam}.mlp")
+        with mce.DeviceGraph(g) as dg:
+            dg.order()
+            F = dg.features()
+            roots = np.nonzero(F[:, 1] > 0)[0]
+            rng = np.random.default_rng(args.seed * 7919 + 1)
+            pick = np.sort(rng.choice(roots, size=min(args.rank_sample, roots.size), replace=False)).astype(np.uint32)
+            t = time.perf_counter()
+            lab = dg.count(level=1, induced=2, worklist=2, order=pick, sub_nodes=True, progress_s=30)
+            y = np.log1p(lab["sub_nodes"][pick].astype(np.float64))
+            pred = mce.mlp_predict(mce.mlp_load(model), F[pick])
+            print(json.dumps({"config": desc, "model": os.path.basename(model), "roots": int(roots.size),
+                              "sampled": int(pick.size), "sample_tree_nodes": lab["tree_nodes"],
+                              "sample_cliques": lab["count"], "enum_s": time.perf_counter() - t,
+                              "spearman_model": spearman(pred, y), "spearman_out_degree": spearman(F[pick, 1], y)}))
+        return
     if args.model is None:
         fam = {"c3": "cl", "c4": "sbm"}.get(args.config, "sbm")
         args.model = os.path.join(ROOT, "mce-gpu_amd", f"cost_model_{fam}.mlp")

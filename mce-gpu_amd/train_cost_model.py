@@ -14,7 +14,13 @@ size), and written in the engine's model format.
 Families: sbm (batches of SBM graphs, n in [64, 4096], 4 blocks, p_in .3,
 p_out .01; --graphs per batch) and chung-lu (beta 2.1, mean degree 24,
 planted cliques of 20..60 vertices, n / 2048 of them as in the 4 M-vertex
-configuration; --n and --planted).
+configuration; --n and --planted, or --scales for several sizes: every
+training seed at every scale, evaluated at --eval-scale).
+
+The committed Chung-Lu model was made with
+
+    python mce-gpu_amd/train_cost_model.py --family chung-lu --scales 18,19,20 \\
+        --train-seeds 1-8 --eval-seed 9 --out mce-gpu_amd/cost_model_cl.mlp
 """
 import argparse
 import importlib.util
@@ -38,11 +44,12 @@ def _engine():
     return mod
 
 
-def make_graph(mce, family, seed, args):
+def make_graph(mce, family, seed, args, n=None):
     if family == "sbm":
         _, g = mce.gen_sbm_batch(args.graphs, 64, 4096, 4, 0.3, 0.01, seed=seed)
         return g
-    return mce.gen_chung_lu(args.n, 2.1, 24.0, args.planted or args.n // 2048, 20, 60, seed=seed)
+    n = n or args.n
+    return mce.gen_chung_lu(n, 2.1, 24.0, args.planted or n // 2048, 20, 60, seed=seed)
 
 
 def labeled(mce, g, worklist, device=0):
@@ -84,6 +91,9 @@ def main():
     ap.add_argument("--graphs", type=int, default=256, help="sbm: graphs per batch")
     ap.add_argument("--n", type=int, default=1 << 18, help="chung-lu: vertices")
     ap.add_argument("--planted", type=int, default=0, help="chung-lu: planted cliques (default n / 2048)")
+    ap.add_argument("--scales", default="", help="chung-lu: comma-separated log2 sizes to train on (overrides --n)")
+    ap.add_argument("--eval-scale", type=int, default=0, help="chung-lu: log2 size of the held-out graph "
+                                                                  "(default: the largest training size)")
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=3e-3)
@@ -94,26 +104,30 @@ def main():
     # labels do not depend on the variant; SBM batches have no heavy tail
     wl = mce.NOWL if args.family == "sbm" else mce.WL
 
+    sizes = [1 << int(k) for k in args.scales.split(",") if k] if args.scales else [args.n]
+    eval_n = (1 << args.eval_scale) if args.eval_scale else max(sizes)
     Fs, ys = [], []
     t0 = time.time()
     for sd in parse_seeds(args.train_seeds):
-        F, y, r = labeled(mce, make_graph(mce, args.family, sd, args), wl)
-        Fs.append(F)
-        ys.append(y)
-        print(f"[train] seed {sd}: {F.shape[0]} subproblems, {r['tree_nodes']} tree nodes", file=sys.stderr,
-              flush=True)
+        for n in sizes:
+            F, y, r = labeled(mce, make_graph(mce, args.family, sd, args, n), wl)
+            Fs.append(F)
+            ys.append(y)
+            print(f"[train] seed {sd} n {n}: {F.shape[0]} subproblems, {r['tree_nodes']} tree nodes",
+                  file=sys.stderr, flush=True)
     F = np.concatenate(Fs)
     y = np.concatenate(ys)
     model = mce.mlp_init(F, seed=args.seed)
     loss = mce.mlp_train(model, F, y, epochs=args.epochs, batch=args.batch, lr=args.lr, seed=args.seed)
     _, train_mse = mce.mlp_grad(model, F, y)
-    Fe, ye, _ = labeled(mce, make_graph(mce, args.family, args.eval_seed, args), wl)
+    Fe, ye, _ = labeled(mce, make_graph(mce, args.family, args.eval_seed, args, eval_n), wl)
     pe = mce.mlp_predict(model, Fe)
     rho = spearman(pe, ye)
     # the plain schedule's implicit predictor: out-degree
     rho_plain = spearman(Fe[:, 1], ye)
     mce.mlp_save(args.out, model)
     print(json.dumps({"family": args.family, "train_seeds": args.train_seeds, "eval_seed": args.eval_seed,
+                      "train_sizes": sizes, "eval_n": eval_n,
                       "train_rows": int(F.shape[0]), "last_epoch_loss": float(loss), "train_mse": float(train_mse),
                       "eval_rows": int(Fe.shape[0]), "spearman_model": rho, "spearman_out_degree": rho_plain,
                       "seconds": time.time() - t0, "out": args.out}))
