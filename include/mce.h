@@ -133,6 +133,10 @@ typedef struct mce_opts {
                               seconds (first-level tickets claimed; with
                               the worker list, tasks outstanding and the
                               busy waves' node heartbeats); 0 = silent      */
+  int lane_universe;       /* small-universe solver: a subtree whose P and X
+                              number at most this many members runs on one
+                              lane, 64 such subtrees per wave at a time
+                              (1..16; 0 = 16; -1 = never)                   */
 
   int schedule;            /* enum mce_schedule                              */
   const uint32_t *order;   /* MCE_SCHED_GIVEN: first-level vertex order      */

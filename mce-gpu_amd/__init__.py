@@ -68,7 +68,7 @@ class Opts(ctypes.Structure):
         ("waves_per_group", c_int), ("groups_per_cu", c_int), ("lds_arena_bytes", c_int),
         ("hbm_arena_bytes", c_int), ("stack_lds_levels", c_int), ("donate_min_p", c_int),
         ("donate_check_nodes", c_int), ("donate_age_us", c_int), ("split_heavy", c_int), ("graph_launch", c_int), ("small_universe", c_int),
-        ("watchdog_us", c_int), ("progress_s", c_int), ("schedule", c_int), ("order", POINTER(c_uint32)),
+        ("watchdog_us", c_int), ("progress_s", c_int), ("lane_universe", c_int), ("schedule", c_int), ("order", POINTER(c_uint32)),
         ("order_len", c_uint64), ("model_path", c_char_p), ("model", POINTER(Mlp)), ("sub_nodes", POINTER(c_uint64)),
         ("sub_cliques", POINTER(c_uint64)),
     ]
@@ -382,7 +382,8 @@ class DeviceGraph:
         buf = ctypes.create_string_buffer(1 << 16)
         _check(lib.mce_autotune(self.handle, byref(base), sample_fraction, byref(best), buf, len(buf)))
         keys = ("worklist", "waves_per_group", "groups_per_cu", "stack_lds_levels", "donate_min_p",
-                "donate_check_nodes", "donate_age_us", "small_universe", "lds_arena_bytes")
+                "donate_check_nodes", "donate_age_us", "small_universe", "lds_arena_bytes",
+                "lane_universe")
         return {k: getattr(best, k) for k in keys}, _json.loads(buf.value.decode() or "[]")
 
     def close(self) -> None:

@@ -84,6 +84,7 @@ int check_opts(const mce_opts *o) {
   if (o->split_heavy < -1) return set_error(MCE_EINVAL, "split_heavy must be -1, 0 or a node count");
   if (o->graph_launch != 0 && o->graph_launch != 1) return set_error(MCE_EINVAL, "graph_launch must be 0 or 1");
   if (o->small_universe < -1 || o->small_universe > 64) return set_error(MCE_EINVAL, "small_universe must be -1..64");
+  if (o->lane_universe < -1 || o->lane_universe > 16) return set_error(MCE_EINVAL, "lane_universe must be -1..16");
   if (o->schedule < MCE_SCHED_PLAIN || o->schedule > MCE_SCHED_GIVEN) return set_error(MCE_EINVAL, "unknown schedule");
   if (o->schedule == MCE_SCHED_GIVEN && !o->order) return set_error(MCE_EINVAL, "given schedule without an order");
   return MCE_OK;
@@ -410,6 +411,7 @@ static int count_impl(mce_graph *h, const mce_opts *opts, mce_result *res, bool 
   cfg.donate_age_us = opts->donate_age_us ? (uint32_t)opts->donate_age_us : 1000;
   cfg.capture = capture || opts->graph_launch;
   cfg.small_max = opts->small_universe < 0 ? 0u : opts->small_universe ? (uint32_t)opts->small_universe : 64u;
+  cfg.lane_max = opts->lane_universe < 0 ? 0u : opts->lane_universe ? (uint32_t)opts->lane_universe : 16u;
   cfg.hbm_bytes = (uint64_t)opts->hbm_arena_bytes;
   cfg.watchdog_us = (uint32_t)opts->watchdog_us;
   cfg.progress_s = (uint32_t)opts->progress_s;
@@ -567,7 +569,7 @@ int mce_autotune(mce_graph *h, const mce_opts *base, double sample_fraction, mce
     return rp[x + 1] - split[x] > rp[y + 1] - split[y];
   });
   struct Cand {
-    int wl, wpg, gpc, stack, minp, check, age, su, lds;
+    int wl, wpg, gpc, stack, minp, check, age, su, lds, lu;
   };
   // the variant (both when the caller leaves it to the tuner), then waves per
   // workgroup x workgroups per CU (each workgroup shares one LDS pool)
@@ -585,7 +587,8 @@ int mce_autotune(mce_graph *h, const mce_opts *base, double sample_fraction, mce
   mce_result r{};
   double best_ms = 1e300;
   Cand bestc{variants[0], base->waves_per_group, base->groups_per_cu, base->stack_lds_levels, base->donate_min_p,
-             base->donate_check_nodes, base->donate_age_us, base->small_universe, base->lds_arena_bytes};
+             base->donate_check_nodes, base->donate_age_us, base->small_universe, base->lds_arena_bytes,
+             base->lane_universe};
   uint64_t ref_count = 0, ref_hash = 0;
   bool have_ref = false;
   auto run = [&](const Cand &cd) -> int {
@@ -599,6 +602,7 @@ int mce_autotune(mce_graph *h, const mce_opts *base, double sample_fraction, mce
     if (cd.age) o.donate_age_us = cd.age;
     o.small_universe = cd.su;
     o.lds_arena_bytes = cd.lds;
+    o.lane_universe = cd.lu;
     memset(&r, 0, sizeof(r));
     const int e = count_impl(h, &o, &r, false);
     if (e) return e;
@@ -609,12 +613,12 @@ int mce_autotune(mce_graph *h, const mce_opts *base, double sample_fraction, mce
     } else if (r.count != ref_count || r.hash != ref_hash) {
       return set_error(MCE_EINTERNAL, "results changed with the block settings");
     }
-    char buf[320];
+    char buf[352];
     snprintf(buf, sizeof(buf), "%s{\"worklist\":%d,\"waves_per_group\":%d,\"groups_per_cu\":%d,"
              "\"stack_lds_levels\":%d,\"donate_min_p\":%d,\"donate_check_nodes\":%d,\"donate_age_us\":%d,"
-             "\"small_universe\":%d,\"lds_arena_bytes\":%d,\"waves\":%u,\"ms\":%.3f}",
+             "\"small_universe\":%d,\"lds_arena_bytes\":%d,\"lane_universe\":%d,\"waves\":%u,\"ms\":%.3f}",
              rep.size() > 1 ? "," : "", cd.wl, cd.wpg, cd.gpc, cd.stack, cd.minp, cd.check, cd.age, cd.su, cd.lds,
-             r.num_waves, r.t_count_ms);
+             cd.lu, r.num_waves, r.t_count_ms);
     rep += buf;
     if (r.t_count_ms < best_ms) {
       best_ms = r.t_count_ms;
@@ -624,14 +628,21 @@ int mce_autotune(mce_graph *h, const mce_opts *base, double sample_fraction, mce
   };
   for (int v : variants)
     for (const auto &sh : shapes) {
-      Cand cd{v, sh[0], sh[1], 0, 0, 0, 0, base->small_universe, 0};
+      Cand cd{v, sh[0], sh[1], 0, 0, 0, 0, base->small_universe, 0, base->lane_universe};
       if ((rc = run(cd))) return rc;
     }
-  // then, one knob at a time from the best so far: the frame levels a
-  // subproblem asks the pool for; the small-universe threshold (which
-  // subtrees run one lane per member); the LDS share per wave (which bit
-  // matrices stay in LDS rather than HBM); then (worker list) the donation
-  // thresholds, which set how many waves a heavy subproblem spreads over
+  // then, one knob at a time from the best so far: the lane solver on or off
+  // (one lane per small subtree pays where those subtrees are large, and its
+  // LDS comes out of the pool); the frame levels a subproblem asks the pool
+  // for; the small-universe threshold (which subtrees run one lane per
+  // member); the LDS share per wave (which bit matrices stay in LDS rather
+  // than HBM); then (worker list) the donation thresholds, which set how many
+  // waves a heavy subproblem spreads over
+  {
+    Cand cd = bestc;
+    cd.lu = bestc.lu < 0 ? 0 : -1;
+    if ((rc = run(cd))) return rc;
+  }
   const Cand shape = bestc;
   for (int st : {8, 48}) {
     Cand cd = shape;
@@ -665,6 +676,7 @@ int mce_autotune(mce_graph *h, const mce_opts *base, double sample_fraction, mce
   best->worklist = bestc.wl;
   best->lds_arena_bytes = bestc.lds;
   best->small_universe = bestc.su;
+  best->lane_universe = bestc.lu;
   best->waves_per_group = bestc.wpg;
   best->groups_per_cu = bestc.gpc;
   best->stack_lds_levels = bestc.stack;

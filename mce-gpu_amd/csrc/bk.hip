@@ -147,6 +147,11 @@ struct BkParams {
   // pinned host words the kernel mirrors its progress into while it runs
   // (word 0: a recent ticket claimed, word 1: heartbeats); nullable
   unsigned long long *host_progress;
+  // lane solver (ls_max = 0: off): largest member count, byte offset of the
+  // waves' LDS regions in the workgroup's LDS, hash-term slots of every wave
+  uint32_t ls_max;
+  uint32_t ls_off;
+  uint64_t *ls_h;
 };
 
 // Posts a progress value to pinned host memory (a system-scope vector store;
@@ -189,6 +194,10 @@ struct WaveAcc {
   uint32_t xq = 0;      // this wave's idle queue (its XCD)
   uint32_t peek = 0;    // another queue sampled at the next poll
   uint64_t t_task = 0;  // realtime at the start of the current task
+  // lane solver: queued subtrees, their largest member count, this wave's
+  // LDS region and hash-term slots
+  uint32_t ls_n = 0, ls_u = 0, ls_lds = 0;
+  uint64_t *ls_h = nullptr;
 };
 
 template <int kStat>
@@ -1325,10 +1334,8 @@ __device__ __forceinline__ void flush_cliques(const BkParams &pr, unsigned long 
 // (the caller flushes before the buffer can overflow)
 __device__ __forceinline__ void keep_clique(const BkParams &pr, unsigned long long *hist_lds, WaveAcc &t,
                                             CliqueBuf &cb, uint32_t size, uint64_t mix) {
-  if (lane_id() == cb.n) {
-    cb.mix = mix;
-    cb.size = size;
-  }
+  cb.mix = wrlane64(cb.mix, cb.n, mix);
+  cb.size = wrlane(cb.size, cb.n, size);
   t.count++;
   t.max_size = max(t.max_size, size);
   cb.n++;
@@ -1368,16 +1375,110 @@ struct MaskOps<uint32_t> {
   __device__ static uint32_t popc(uint32_t x) { return (uint32_t)__builtin_popcount(x); }
   __device__ static uint32_t ctz(uint32_t x) { return (uint32_t)__builtin_ctz(x); }
   __device__ static uint32_t rd(uint32_t v, uint32_t l) { return rdlane(v, l); }
-  // lanes 32..63 hold no member
-  __device__ static bool bit(uint32_t m, uint32_t L) { return L < 32 && ((m >> (L & 31)) & 1u); }
+  // as a lane mask: lanes 32..63 hold no member
+  __device__ static uint64_t lanes(uint32_t m) { return m; }
+  __device__ static uint32_t wr(uint32_t f, uint32_t l, uint32_t v) { return wrlane(f, l, v); }
 };
 template <>
 struct MaskOps<uint64_t> {
   __device__ static uint32_t popc(uint64_t x) { return popc64(x); }
   __device__ static uint32_t ctz(uint64_t x) { return ctz64(x); }
   __device__ static uint64_t rd(uint64_t v, uint32_t l) { return rdlane64(v, l); }
-  __device__ static bool bit(uint64_t m, uint32_t L) { return (m >> L) & 1ull; }
+  __device__ static uint64_t lanes(uint64_t m) { return m; }
+  __device__ static uint64_t wr(uint64_t f, uint32_t l, uint64_t v) { return wrlane64(f, l, v); }
 };
+
+// ---------------------------------------------------------------------------
+// Lane solver: subtrees whose P and X number at most kLsMax members run one
+// per lane, up to kLsSlots of them at a time. The small-universe solver
+// queues such a subtree (ls_put) instead of descending into it: the members
+// are renumbered 0..k-1 in their order, so pivots (largest |P & N(u)|, ties to
+// the lowest member) and child order (lowest member first) are unchanged, and
+// the lane visits the nodes the wave would have visited. A full queue, and
+// the end of every bitmap subproblem, runs the queue (ls_run).
+//
+// Per wave: in LDS only the slots' rows (16-bit masks over the members, read
+// at every node; 2 KiB, so that the bit matrices' LDS pool keeps most of its
+// room); in device memory the members' hash terms (kLsMax 64-bit words per
+// slot), one 16-byte header per slot (P | k << 16, |R| at the subtree's root,
+// R's hash sum) and kLsLevels frames per slot ((P | X << 16), level-major so
+// that lanes at one depth touch consecutive words).
+constexpr uint32_t kLsMax = 16, kLsSlots = 64;
+// A pushed level has |P| >= 3 and every level has one member fewer in P than
+// its parent, so at most kLsMax - 3 levels are pushed.
+constexpr uint32_t kLsLevels = kLsMax - 3;
+constexpr uint32_t kLsBytes = kLsSlots * kLsMax * 2;  // LDS per wave
+constexpr uint32_t kLsHdrOff = kLsSlots * kLsMax * 8, kLsFrOff = kLsHdrOff + kLsSlots * 16,
+                   kLsGlobalBytes = kLsFrOff + kLsLevels * kLsSlots * 4;  // device memory per wave
+// the small-universe solver starts only with this many free slots
+constexpr uint32_t kLsReserve = 16;
+// busy lanes at which the wave takes over the rest of their subtrees
+constexpr uint32_t kLsDrain = 6;
+typedef __attribute__((address_space(3))) uint16_t lds_u16;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
+
+struct LsOut {
+  uint64_t count, nodes, hash;
+  uint32_t max_size;
+};
+
+// Queues the subtree of a small-universe node: K = its members (P | X) over
+// the solver's numbering, Pk = its P, rb = |R| there, mix = R's hash sum; the
+// calling lane i holds member i's row M and hash term hs. Wave-uniform call
+// with a free slot.
+template <typename Mk>
+__device__ __forceinline__ void ls_put(WaveAcc &wa, Mk M, uint64_t hs, Mk K, Mk Pk, uint32_t rb, uint64_t mix) {
+  using O = MaskOps<Mk>;
+  const uint32_t L = lane_id(), j = wa.ls_n, nk = O::popc(K);
+  const bool in = lane_in(O::lanes(K));
+  const uint32_t k = mbcnt((uint64_t)K);  // the member's new index
+  uint32_t row;
+  if ((K & (K + 1)) == 0) {
+    row = (uint32_t)(M & K);  // members 0..k-1 keep their numbers
+  } else {
+    row = 0;
+    uint32_t t = 0;
+    for (Mk kk = K; kk; kk &= kk - 1, ++t) row |= (uint32_t)((M >> O::ctz(kk)) & 1u) << t;
+  }
+  lds_u16 *rows = (lds_u16 *)(uintptr_t)wa.ls_lds;
+  if (in) {
+    rows[j * kLsMax + k] = (uint16_t)row;
+    wa.ls_h[j * kLsMax + k] = hs;
+  }
+  // P over the new indices: every member's P flag moves to lane k
+  const uint32_t f = (uint32_t)__builtin_amdgcn_ds_permute((int)((in ? k : kWave - 1) * 4),
+                                                           lane_in(O::lanes(Pk)) ? 1 : 0);
+  const uint32_t pk = (uint32_t)ballot(L < nk && f != 0);
+  if (L == 0) {
+    u32x4 *hdr = (u32x4 *)((uint8_t *)wa.ls_h + kLsHdrOff);
+    hdr[j] = u32x4{pk | nk << 16, rb, (uint32_t)mix, (uint32_t)(mix >> 32)};
+  }
+  wa.ls_n = j + 1;
+  wa.ls_u = max(wa.ls_u, nk);
+}
+
+// Pivot of a lane's node: the member u of P | X with the most members of P
+// among its neighbours, ties to the lowest u. False when that count is |P|
+// (no child). R: the lane's 16 rows; um: members to scan (uniform).
+__device__ __forceinline__ bool ls_pick(const lds_u16 *R, uint32_t P, uint32_t X, uint32_t um, uint32_t &piv) {
+  const u32x4 w0 = *(const lds_u32x4 *)R;
+  u32x4 w1 = {0, 0, 0, 0};
+  if (um > 8) w1 = *(const lds_u32x4 *)(R + 8);
+  const uint32_t PX = P | X, Ph = P << 16;
+  uint32_t best = 0;
+#pragma unroll
+  for (uint32_t u = 0; u < kLsMax; ++u) {
+    if (u >= um) break;
+    const uint32_t w = u < 8 ? w0[u >> 1] : w1[(u - 8) >> 1];
+    // key: count << 5 | (31 - u), so the maximum picks the lowest u of the best count
+    const uint32_t c = (uint32_t)__builtin_popcount(w & ((u & 1) ? Ph : P));
+    const uint32_t key = (c << 5) | (31u - u);
+    best = max(best, ((PX >> u) & 1u) ? key : 0u);
+  }
+  piv = 31u - (best & 31u);
+  return best != 0 && (best >> 5) != (uint32_t)__builtin_popcount(P);
+}
 
 // The small-universe solver's loop over a renumbered universe: lane i holds
 // member i's row M over the members and its hash term hs; lP / lX are the
@@ -1385,13 +1486,12 @@ struct MaskOps<uint64_t> {
 template <typename Mk, int kStat>
 __device__ __forceinline__ void small_loop(const BkParams &pr, Mk M, uint64_t hs, Mk lP, Mk lX, uint64_t mix,
                                            uint32_t rbase, unsigned long long *hist_lds, WaveAcc &wa,
-                                           Clock<kStat> &clk) {
+                                           Clock<kStat> &clk, int piv0 = -1) {
   using O = MaskOps<Mk>;
-  const uint32_t L = lane_id();
   // pivot over P | X (the lowest member among the best scores): false when
   // it covers P (no child)
   auto pick = [&](Mk P, Mk X, uint32_t &pv) -> bool {
-    const uint32_t key = O::bit(P | X, L) ? O::popc(M & P) + 1u : 0u;
+    const uint32_t key = lane_in(O::lanes(P | X)) ? O::popc(M & P) + 1u : 0u;
     const uint32_t mx = wave_max(key);
     pv = ctz64(ballot(key == mx));
     return mx != 0 && mx - 1u != O::popc(P);
@@ -1399,7 +1499,8 @@ __device__ __forceinline__ void small_loop(const BkParams &pr, Mk M, uint64_t hs
   clk.to(kBdPivot);
   Mk P = lP, X = lX;
   uint32_t piv;
-  if (!pick(P, X, piv)) return;
+  if (piv0 >= 0) piv = (uint32_t)piv0;  // a node whose pivot is known: its remaining children
+  else if (!pick(P, X, piv)) return;
   Mk Np = O::rd(M, piv);
   uint32_t d = 0;
   // frames: lane d of each register holds level d
@@ -1477,16 +1578,20 @@ __device__ __forceinline__ void small_loop(const BkParams &pr, Mk M, uint64_t hs
       if ((Xn & (piv_a ? Mb : Ma)) == 0) keep_clique(pr, hist_lds, t, kb, rbase + d + 2, cmix + (piv_a ? hb : ha));
       continue;
     }
+    if (pr.ls_max && O::popc(Pn | Xn) <= pr.ls_max && wa.ls_n < kLsSlots) {
+      clk.to(kBdBranch);
+      ls_put<Mk>(wa, M, hs, Pn | Xn, Pn, rbase + d + 1, cmix);
+      continue;
+    }
     clk.to(kBdPivot);
     uint32_t npiv;
     if (!pick(Pn, Xn, npiv)) continue;
     clk.to(kBdBranch);
-    const bool here = L == d;
-    fP = here ? P : fP;
-    fX = here ? X : fX;
-    fMl = here ? (uint32_t)mix : fMl;
-    fMh = here ? (uint32_t)(mix >> 32) : fMh;
-    fV = here ? piv : fV;
+    fP = O::wr(fP, d, P);
+    fX = O::wr(fX, d, X);
+    fMl = wrlane(fMl, d, (uint32_t)mix);
+    fMh = wrlane(fMh, d, (uint32_t)(mix >> 32));
+    fV = wrlane(fV, d, piv);
     ++d;
     P = Pn;
     X = Xn;
@@ -1502,10 +1607,193 @@ __device__ __forceinline__ void small_loop(const BkParams &pr, Mk M, uint64_t hs
   wa.max_size = max(wa.max_size, t.max_size);
 }
 
+// Runs the n queued subtrees, lane j the subtree of slot j, node for node as
+// the small-universe solver would (the |P| <= 2 shortcuts included). um: the
+// largest member count among them. Once at most kLsDrain lanes are still
+// busy, the wave takes over: each such lane's remaining work (the open
+// children of its current node and of every frame below it) runs through
+// the small-universe solver's loop, which resumes a node at its known pivot,
+// so one large subtree does not hold the other lanes idle. Returns the
+// totals.
+__device__ __noinline__ LsOut ls_run(uint32_t n, uint32_t um, uint32_t lds, const uint64_t *hg, uint32_t hist_lds_addr,
+                                     uint32_t hist_lds_n, unsigned long long *hist, uint32_t hist_len) {
+  n = uniform(n);
+  um = uniform(um);
+  lds = uniform(lds);
+  hist_lds_addr = uniform(hist_lds_addr);
+  hist_lds_n = uniform(hist_lds_n);
+  hist_len = uniform(hist_len);
+  const uint32_t L = lane_id();
+  // the queue's device-memory writes (hash terms, headers) precede these reads
+  __builtin_amdgcn_s_waitcnt(0);
+  const lds_u16 *R = (const lds_u16 *)(uintptr_t)lds + L * kLsMax;
+  const glb_u32 *hdr = (const glb_u32 *)((const uint8_t *)hg + kLsHdrOff) + 4 * L;
+  glb_u32 *fr = (glb_u32 *)((uint8_t *)hg + kLsFrOff) + L;
+  typedef __attribute__((address_space(3))) unsigned long long lds_ull;
+  lds_ull *hist_lds = (lds_ull *)(uintptr_t)hist_lds_addr;
+  const glb_u64 *H = (const glb_u64 *)hg + L * kLsMax;
+  uint32_t cnt = 0, nodes = 0, maxs = 0;
+  uint64_t hacc = 0;
+  auto emit = [&](uint32_t size, uint64_t m) {
+    ++cnt;
+    maxs = max(maxs, size);
+    hacc += clique_hash(m, size);
+    if (size < hist_lds_n) __hip_atomic_fetch_add(&hist_lds[size], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else atomicAdd(&hist[min(size, hist_len - 1)], 1ull);
+  };
+  uint32_t P = 0, X = 0, piv = 0, d = 0, rb = 0, Np = 0;
+  uint64_t mix = 0, path = 0, pivs = 0;
+  bool act = L < n;
+  uint32_t h0 = 0;
+  if (act) {
+    h0 = hdr[0];
+    P = h0 & 0xFFFFu;
+    X = ((1u << (h0 >> 16)) - 1u) & ~P;
+    rb = hdr[1];
+    mix = (uint64_t)hdr[2] | (uint64_t)hdr[3] << 32;
+    act = ls_pick(R, P, X, um, piv);
+    if (act) Np = R[piv];
+  }
+  while (true) {
+    if (popc64(ballot(act)) <= kLsDrain) break;
+    if (!act) continue;
+    const uint32_t B = P & ~Np;
+    if (B == 0) {
+      if (d == 0) {
+        act = false;
+        continue;
+      }
+      --d;
+      const uint32_t f = fr[d * kLsSlots], sh = 4 * d;
+      P = f & 0xFFFFu;
+      X = f >> 16;
+      mix -= H[(uint32_t)(path >> sh) & 15u];
+      piv = (uint32_t)(pivs >> sh) & 15u;
+      Np = R[piv];
+      continue;
+    }
+    const uint32_t c = (uint32_t)__builtin_ctz(B), cb = 1u << c;
+    P ^= cb;
+    X |= cb;
+    const uint32_t Mc = R[c];
+    const uint32_t Pn = P & Mc, Xn = X & Mc;
+    ++nodes;
+    const uint64_t cmix = mix + H[c];
+    if (Pn == 0) {
+      if (Xn == 0) emit(rb + d + 1, cmix);
+      continue;
+    }
+    const uint32_t np = (uint32_t)__builtin_popcount(Pn);
+    if (np <= 2) {
+      const uint32_t a = (uint32_t)__builtin_ctz(Pn), Ma = R[a];
+      const uint64_t ha = H[a];
+      if (np == 1) {
+        if ((Xn & Ma) == 0) {
+          ++nodes;
+          emit(rb + d + 2, cmix + ha);
+        }
+        continue;
+      }
+      const uint32_t b = (uint32_t)__builtin_ctz(Pn & (Pn - 1)), Mb = R[b];
+      const uint64_t hb = H[b];
+      if (Xn & Ma & Mb) continue;  // an X member covers P
+      if ((Ma >> b) & 1u) {
+        nodes += 2;
+        emit(rb + d + 3, cmix + ha + hb);
+        continue;
+      }
+      const uint32_t xs = Xn & (Ma | Mb);
+      if (xs == 0) {
+        nodes += 2;
+        emit(rb + d + 2, cmix + ha);
+        emit(rb + d + 2, cmix + hb);
+        continue;
+      }
+      // the pivot is the lowest X member adjacent to one of a, b; the child
+      // is the other one
+      const bool piv_a = (Ma >> __builtin_ctz(xs)) & 1u;
+      ++nodes;
+      if ((Xn & (piv_a ? Mb : Ma)) == 0) emit(rb + d + 2, cmix + (piv_a ? hb : ha));
+      continue;
+    }
+    uint32_t npiv;
+    if (!ls_pick(R, Pn, Xn, um, npiv)) continue;
+    fr[d * kLsSlots] = P | X << 16;
+    const uint32_t sh = 4 * d;
+    path = (path & ~(15ull << sh)) | (uint64_t)c << sh;
+    pivs = (pivs & ~(15ull << sh)) | (uint64_t)piv << sh;
+    ++d;
+    P = Pn;
+    X = Xn;
+    mix = cmix;
+    piv = npiv;
+    Np = R[piv];
+  }
+  LsOut o;
+  o.count = wave_sum(cnt);
+  o.nodes = wave_sum(nodes);
+  o.hash = wave_add64(hacc);
+  o.max_size = wave_max(maxs);
+  // the busy lanes' remaining work, on the whole wave
+  BkParams q{};
+  q.hist = hist;
+  q.hist_len = hist_len;
+  q.hist_lds = hist_lds_n;
+  WaveAcc w{};
+  Clock<kStatNone> clk;
+  const lds_u16 *rows = (const lds_u16 *)(uintptr_t)lds;
+  const glb_u32 *frs = (const glb_u32 *)((const uint8_t *)hg + kLsFrOff);
+  for (uint64_t left = ballot(act); left; left &= left - 1) {
+    const uint32_t j = ctz64(left);
+    // slot j's rows and hash terms, member k on lane k
+    const uint32_t nm = rdlane(h0, j) >> 16;
+    const uint32_t M = L < nm ? rows[j * kLsMax + L] : 0u;
+    const uint64_t hs = L < nm ? ((const glb_u64 *)hg)[j * kLsMax + L] : 0ull;
+    uint32_t Pk = rdlane(P, j), Xk = rdlane(X, j), pk = rdlane(piv, j);
+    uint64_t mk = rdlane64(mix, j);
+    const uint32_t dj = rdlane(d, j), rbj = rdlane(rb, j);
+    const uint64_t pathj = rdlane64(path, j), pivsj = rdlane64(pivs, j);
+    for (uint32_t k = dj + 1; k-- > 0;) {
+      if (k < dj) {
+        const uint32_t f = frs[k * kLsSlots + j], sh = 4 * k;
+        Pk = f & 0xFFFFu;
+        Xk = f >> 16;
+        mk -= rdlane64(hs, (uint32_t)(pathj >> sh) & 15u);
+        pk = (uint32_t)(pivsj >> sh) & 15u;
+      }
+      small_loop<uint32_t, kStatNone>(q, M, hs, Pk, Xk, mk, rbj + k, (unsigned long long *)hist_lds, w, clk,
+                                      (int)pk);
+    }
+  }
+  o.count += w.count;
+  o.nodes += w.nodes;
+  o.hash += w.hash;
+  o.max_size = max(o.max_size, w.max_size);
+  return o;
+}
+
+// Runs the wave's queue, if any, and adds its totals to the wave's.
+__device__ __forceinline__ void ls_flush(const BkParams &pr, unsigned long long *hist_lds, WaveAcc &wa) {
+  if (wa.ls_n == 0) return;
+  const LsOut o = ls_run(wa.ls_n, wa.ls_u, wa.ls_lds, wa.ls_h, (uint32_t)(uintptr_t)hist_lds, pr.hist_lds, pr.hist,
+                         pr.hist_len);
+  wa.count += o.count;
+  wa.sub_cliques += o.count;
+  wa.nodes += o.nodes;
+  wa.hash += o.hash;
+  wa.max_size = max(wa.max_size, o.max_size);
+  wa.ls_n = 0;
+  wa.ls_u = 0;
+}
+
 template <bool kWL, int kStat, bool kLA, bool kLX>
 __device__ __forceinline__ void solve_small(const BkParams &pr, const Sub &s, uint64_t Pw, uint64_t Xw, uint64_t Ew,
                                             uint64_t mix, uint32_t rbase, unsigned long long *hist_lds, WaveAcc &wa,
                                             Clock<kStat> &clk) {
+  if (pr.ls_max && wa.ls_n > kLsSlots - kLsReserve) {
+    clk.to(kBdSetOps);
+    ls_flush(pr, hist_lds, wa);
+  }
   const uint32_t L = lane_id(), W = s.W;
   clk.to(kBdInduced);
   // lane i <- code of member i: each nonzero word pushes its set bits to
@@ -1564,6 +1852,11 @@ __device__ __forceinline__ void solve_small(const BkParams &pr, const Sub &s, ui
   }
   const uint64_t hs = valid && !out ? ld64<kLA>(s.H, g) : 0ull;
 
+  if (pr.ls_max && nm <= pr.ls_max) {
+    clk.to(kBdBranch);
+    ls_put<uint32_t>(wa, (uint32_t)M, hs, (1u << nm) - 1u, (uint32_t)lP, rbase, mix);
+    return;
+  }
   if (nm <= 32)
     small_loop<uint32_t, kStat>(pr, (uint32_t)M, hs, (uint32_t)lP, (uint32_t)lX, mix, rbase, hist_lds, wa, clk);
   else
@@ -1728,6 +2021,9 @@ __device__ __forceinline__ void solve_bits(const BkParams &pr, Sub &s, uint64_t 
   if (s.where == 3) solve_bits_t<kWL, kStat, true, true>(pr, s, cP, cX, cE, hist_lds, wa, clk);
   else if (s.where == 1) solve_bits_t<kWL, kStat, true, false>(pr, s, cP, cX, cE, hist_lds, wa, clk);
   else solve_bits_t<kWL, kStat, false, false>(pr, s, cP, cX, cE, hist_lds, wa, clk);
+  // the subproblem's counts are complete once its queued subtrees have run
+  clk.to(kBdSetOps);
+  ls_flush(pr, hist_lds, wa);
 }
 
 // Intersection of ascending a[0,na) and b[0,nb) written to out (ascending);
@@ -1830,6 +2126,10 @@ __global__ __launch_bounds__(max_wpg_of(kWL) * kWave, waves_per_simd_of(kWL)) vo
   const uint32_t gw = blockIdx.x * pr.wpg + wig;
   uint8_t *hbm = pr.hbm + (uint64_t)gw * pr.hbm_bytes;
   WaveAcc wa;
+  if (pr.ls_max) {
+    wa.ls_lds = (uint32_t)(uintptr_t)(smem + pr.ls_off + wig * kLsBytes);
+    wa.ls_h = pr.ls_h + (uint64_t)gw * (kLsGlobalBytes / 8);
+  }
   wa.xq = (hw_cu_id() >> 8) % kQueues;
   wa.peek = wa.xq;
   Clock<kStat> clk;
@@ -2352,10 +2652,15 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
     wpg = std::max<uint32_t>(1, waves_cu / groups_cu);
   }
   wpg = std::max<uint32_t>(1, std::min<uint32_t>(wpg, max_wpg));
-  size_t pool = cfg.lds_bytes ? (size_t)cfg.lds_bytes * wpg
-                              : lds_cu / groups_cu - hist_bytes - kPoolHeader;
+  // the lane solver's regions (one per wave) follow the pool, 64-byte aligned
+  const uint32_t ls_max = std::min<uint32_t>(cfg.lane_max, kLsMax);
+  const size_t ls_total = ls_max ? (size_t)kLsBytes * wpg + 64 : 0;
+  const size_t fixed = hist_bytes + kPoolHeader + ls_total, share = lds_cu / groups_cu;
+  size_t pool = cfg.lds_bytes ? (size_t)cfg.lds_bytes * wpg : share > fixed + kGranule ? share - fixed : kGranule;
+  pool = std::min<size_t>(pool, lds_cu - fixed);
   pool = std::min<size_t>(pool, (size_t)kPoolWords * 64 * kGranule) / kGranule * kGranule;
-  const size_t lds = hist_bytes + kPoolHeader + pool;
+  const size_t ls_off = ls_max ? (hist_bytes + kPoolHeader + pool + 63) & ~(size_t)63 : 0;
+  const size_t lds = ls_max ? ls_off + (size_t)kLsBytes * wpg : hist_bytes + kPoolHeader + pool;
   if (lds > lds_cu) return set_error(MCE_EINVAL, "LDS per workgroup %zu exceeds 160 KiB", lds);
 
   // every device buffer of this call: carved from the caller's scratch
@@ -2455,6 +2760,8 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   unsigned long long *wave_val = nullptr, *wave_bd = nullptr;
   uint32_t *wave_cu = nullptr;
   if (arena && (rc = dmalloc((void **)&hbm, arena * waves, kWsArena))) return rc;
+  uint64_t *ls_h = nullptr;
+  if (ls_max && (rc = dmalloc((void **)&ls_h, (uint64_t)kLsGlobalBytes * waves, kWsLane))) return rc;
   if (hneed[1] && (rc = dmalloc((void **)&overflow, sizeof(uint64_t) * hneed[1], kWsOverflow))) return rc;
   if (wave_stats) {
     if ((rc = dmalloc((void **)&wave_val, sizeof(unsigned long long) * waves, -1))) return rc;
@@ -2477,6 +2784,9 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   pr.want_levels = cfg.stack_lds_levels ? cfg.stack_lds_levels : 24;
   pr.small_max = std::min<uint32_t>(cfg.small_max, kSmallMax);
   pr.small_pmax = kSmallMaxPWl;
+  pr.ls_max = ls_max;
+  pr.ls_off = (uint32_t)ls_off;
+  pr.ls_h = ls_h;
   pr.watchdog_ticks = cfg.watchdog_us ? (uint64_t)cfg.watchdog_us * 100 : kWatchdogTicks;
   pr.hbm = hbm;
   pr.hbm_bytes = arena;

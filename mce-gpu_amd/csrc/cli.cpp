@@ -49,6 +49,7 @@ void usage() {
           "\n    --donate-check N      tree nodes between idle-worker polls"
           "\n    --donate-age US       a task donates only after running this long"
           "\n    --small-universe N    nodes with |P|+|X| <= N (1..64) run one lane per member; -1 = never"
+          "\n    --lane-universe N     subtrees with |P|+|X| <= N (1..16) run one per lane; -1 = never"
           "\n    --hbm-arena BYTES     HBM arena cap per wave (default 16 MiB; larger subproblems run in a second launch)"
           "\n    --watchdog-us US      worker list: give up after this long without progress (default 120 s)"
           "\n    --progress S          print the enumeration's progress counters to stderr every S seconds"
@@ -106,6 +107,7 @@ int main(int argc, char **argv) {
                                      {"hbm-arena", required_argument, 0, 11},
                                      {"watchdog-us", required_argument, 0, 12},
                                      {"progress", required_argument, 0, 13},
+                                     {"lane-universe", required_argument, 0, 14},
                                      {0, 0, 0, 0}};
   int opt;
   while ((opt = getopt_long(argc, argv, "g:r:d:m:p:i:w:h", longopts, nullptr)) >= 0) {
@@ -168,6 +170,7 @@ int main(int argc, char **argv) {
       case 11: opts.hbm_arena_bytes = atoi(optarg); break;
       case 12: opts.watchdog_us = atoi(optarg); break;
       case 13: opts.progress_s = atoi(optarg); break;
+      case 14: opts.lane_universe = atoi(optarg); break;
       default:
         fprintf(stderr, "\nUnrecognized option!\n");
         usage();

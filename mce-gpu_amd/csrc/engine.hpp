@@ -56,6 +56,7 @@ enum WsSlot : int {
   kWsRing,
   kWsIdle,
   kWsMbox,
+  kWsLane,      // lane solver: hash terms, headers and frames of every wave
   kWsSlots
 };
 struct Workspace {
@@ -112,6 +113,7 @@ struct BkLaunch {
   uint32_t donate_check_nodes = 64;
   uint32_t donate_age_us = 1000;
   uint32_t small_max = 64;  // P + X + outside rows at which a node goes to the small-universe solver (0 = never)
+  uint32_t lane_max = 16;   // P + X at which the small-universe solver queues a subtree for the lane solver (0 = never)
   uint32_t watchdog_us = 0;  // worker list: idle time without global progress before giving up (0 = 120 s)
   uint32_t progress_s = 0;   // > 0: print the launch's own progress counters to stderr this often
   bool capture = false;  // run the enumeration launch as a captured hipGraph

@@ -105,6 +105,8 @@ extern "C" int mce_graph_count_noalloc(mce_graph *h, const mce_opts *opts, mce_r
   cfg.lds_bytes = (uint32_t)opts->lds_arena_bytes;
   cfg.stack_lds_levels = (uint32_t)opts->stack_lds_levels;
   cfg.small_max = opts->small_universe < 0 ? 0u : opts->small_universe ? (uint32_t)opts->small_universe : 64u;
+  // the lane solver's per-wave device memory (12.25 KiB) is left to the arenas
+  cfg.lane_max = 0;
   cfg.hbm_bytes = (uint64_t)opts->hbm_arena_bytes;
   cfg.capture = false;
   BkOut out;

@@ -8,6 +8,10 @@ extern "C" __device__ __attribute__((const)) unsigned int __ockl_wfred_add_u32(u
 extern "C" __device__ __attribute__((const)) unsigned int __ockl_wfred_max_u32(unsigned int);
 extern "C" __device__ __attribute__((const)) unsigned int __ockl_wfred_min_u32(unsigned int);
 extern "C" __device__ __attribute__((const)) unsigned int __ockl_wfred_or_u32(unsigned int);
+// v_writelane_b32: the LLVM intrinsic, which clang does not expose as a builtin
+extern "C" __device__ __attribute__((const)) unsigned int mce_llvm_writelane(unsigned int v, unsigned int lane,
+                                                                             unsigned int old)
+    __asm("llvm.amdgcn.writelane.i32");
 
 namespace mce {
 
@@ -30,6 +34,19 @@ __device__ __forceinline__ uint64_t rdlane64(uint64_t v, uint32_t l) {
   const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), l);
   return ((uint64_t)hi << 32) | lo;
 }
+
+// `old` with lane l replaced by the uniform value v (one v_writelane per
+// 32 bits, updated in place; l and v must be wave-uniform).
+__device__ __forceinline__ uint32_t wrlane(uint32_t old, uint32_t l, uint32_t v) { return mce_llvm_writelane(v, l, old); }
+__device__ __forceinline__ uint64_t wrlane64(uint64_t old, uint32_t l, uint64_t v) {
+  const uint32_t lo = mce_llvm_writelane((uint32_t)v, l, (uint32_t)old);
+  const uint32_t hi = mce_llvm_writelane((uint32_t)(v >> 32), l, (uint32_t)(old >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Bit `lane_id()` of the uniform mask m, with m used as the lane mask itself
+// (no per-lane shift and compare).
+__device__ __forceinline__ bool lane_in(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
 
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ uint64_t uniform64(uint64_t v) {

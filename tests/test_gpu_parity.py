@@ -112,6 +112,32 @@ def test_small_universe_thresholds(mce, oracle, small):
             lds_arena_bytes=1024)
 
 
+@pytest.mark.parametrize("lane", [-1, 4, 16])
+def test_lane_universe_thresholds(mce, oracle, lane):
+    # subtrees with |P| + |X| <= lane run one per lane (-1: never; 4: only
+    # the smallest; 16: every one the small-universe solver meets), with and
+    # without the worker list, with per-subproblem labels: a Chung-Lu graph
+    # with planted cliques queues thousands of subtrees per heavy subproblem
+    # (full queues, runs in the middle of a subtree), the dense random graph
+    # few large ones, and a tiny LDS arena moves the bit matrices to HBM
+    g = mce.gen_chung_lu(1 << 14, 2.1, 24.0, 8, 20, 40, seed=3)
+    ref = oracle.mce(g, level=1, ipx=True, sub_nodes=True, sub_cliques=True)
+    with mce.DeviceGraph(g) as dg:
+        dg.order()
+        for wl in (1, 2):
+            print(f"case lane={lane} chung-lu 2^14 wl={wl}", flush=True)
+            res = dg.count(level=1, induced=2, worklist=wl, lane_universe=lane, sub_nodes=True, sub_cliques=True)
+            assert_same(res, ref, (lane, wl))
+            np.testing.assert_array_equal(res["sub_nodes"], ref["sub_nodes"])
+            np.testing.assert_array_equal(res["sub_cliques"], ref["sub_cliques"])
+    for n, p, seed in [(200, 0.5, 1), (120, 0.3, 2)]:
+        print(f"case lane={lane} n={n} p={p}", flush=True)
+        run_all(mce, oracle, rand_graph(mce, n, p, seed), f"lane{lane}-{n}", lane_universe=lane)
+    print(f"case lane={lane} hbm arena", flush=True)
+    run_all(mce, oracle, rand_graph(mce, 220, 0.55, 7), f"lane{lane}-hbm", lane_universe=lane,
+            lds_arena_bytes=1024, worklist=2)
+
+
 def test_hbm_arena_and_frame_spill(mce, oracle):
     # a tiny LDS arena forces the HBM arena and HBM stack frames
     g = rand_graph(mce, 220, 0.55, 7)
