@@ -68,7 +68,11 @@ enum Acc : uint32_t {
   kAccN
 };
 
-enum Stat : int { kStatNone = 0, kStatLb = 1, kStatDonor = 2, kStatBd = 3 };
+// Kernel instances: no statistics, the three eval variants, and no
+// statistics with the lane solver compiled in (kept out of the other
+// instances, whose register allocation it would otherwise share).
+enum Stat : int { kStatNone = 0, kStatLb = 1, kStatDonor = 2, kStatBd = 3, kStatLane = 4 };
+constexpr bool lane_of(int stat) { return stat == kStatLane; }
 enum Bd : int { kBdOther = 0, kBdInduced, kBdQueue, kBdPivot, kBdSetOps, kBdMaximal, kBdBranch };
 
 // One idle queue per XCD (its head and tail each on a line of their own from
@@ -1578,7 +1582,7 @@ __device__ __forceinline__ void small_loop(const BkParams &pr, Mk M, uint64_t hs
       if ((Xn & (piv_a ? Mb : Ma)) == 0) keep_clique(pr, hist_lds, t, kb, rbase + d + 2, cmix + (piv_a ? hb : ha));
       continue;
     }
-    if (pr.ls_max && O::popc(Pn | Xn) <= pr.ls_max && wa.ls_n < kLsSlots) {
+    if (lane_of(kStat) && O::popc(Pn | Xn) <= pr.ls_max && wa.ls_n < kLsSlots) {
       clk.to(kBdBranch);
       ls_put<Mk>(wa, M, hs, Pn | Xn, Pn, rbase + d + 1, cmix);
       continue;
@@ -1790,7 +1794,7 @@ template <bool kWL, int kStat, bool kLA, bool kLX>
 __device__ __forceinline__ void solve_small(const BkParams &pr, const Sub &s, uint64_t Pw, uint64_t Xw, uint64_t Ew,
                                             uint64_t mix, uint32_t rbase, unsigned long long *hist_lds, WaveAcc &wa,
                                             Clock<kStat> &clk) {
-  if (pr.ls_max && wa.ls_n > kLsSlots - kLsReserve) {
+  if (lane_of(kStat) && wa.ls_n > kLsSlots - kLsReserve) {
     clk.to(kBdSetOps);
     ls_flush(pr, hist_lds, wa);
   }
@@ -1852,7 +1856,7 @@ __device__ __forceinline__ void solve_small(const BkParams &pr, const Sub &s, ui
   }
   const uint64_t hs = valid && !out ? ld64<kLA>(s.H, g) : 0ull;
 
-  if (pr.ls_max && nm <= pr.ls_max) {
+  if (lane_of(kStat) && nm <= pr.ls_max) {
     clk.to(kBdBranch);
     ls_put<uint32_t>(wa, (uint32_t)M, hs, (1u << nm) - 1u, (uint32_t)lP, rbase, mix);
     return;
@@ -2022,8 +2026,10 @@ __device__ __forceinline__ void solve_bits(const BkParams &pr, Sub &s, uint64_t 
   else if (s.where == 1) solve_bits_t<kWL, kStat, true, false>(pr, s, cP, cX, cE, hist_lds, wa, clk);
   else solve_bits_t<kWL, kStat, false, false>(pr, s, cP, cX, cE, hist_lds, wa, clk);
   // the subproblem's counts are complete once its queued subtrees have run
-  clk.to(kBdSetOps);
-  ls_flush(pr, hist_lds, wa);
+  if constexpr (lane_of(kStat)) {
+    clk.to(kBdSetOps);
+    ls_flush(pr, hist_lds, wa);
+  }
 }
 
 // Intersection of ascending a[0,na) and b[0,nb) written to out (ascending);
@@ -2126,7 +2132,7 @@ __global__ __launch_bounds__(max_wpg_of(kWL) * kWave, waves_per_simd_of(kWL)) vo
   const uint32_t gw = blockIdx.x * pr.wpg + wig;
   uint8_t *hbm = pr.hbm + (uint64_t)gw * pr.hbm_bytes;
   WaveAcc wa;
-  if (pr.ls_max) {
+  if (lane_of(kStat)) {
     wa.ls_lds = (uint32_t)(uintptr_t)(smem + pr.ls_off + wig * kLsBytes);
     wa.ls_h = pr.ls_h + (uint64_t)gw * (kLsGlobalBytes / 8);
   }
@@ -2520,6 +2526,12 @@ int dispatch_stat(int stat, bool occ, int *nb, const BkParams &pr, uint32_t grou
     case kStatBd:
       return occ ? occupancy_one<kIPX, kWL, kStatBd>(nb, wpg, lds)
                  : launch_one<kIPX, kWL, kStatBd>(pr, groups, wpg, lds, s);
+    case kStatLane:
+      // the lane solver lives in the bitmap path, which only IPX takes
+      if constexpr (kIPX)
+        return occ ? occupancy_one<kIPX, kWL, kStatLane>(nb, wpg, lds)
+                   : launch_one<kIPX, kWL, kStatLane>(pr, groups, wpg, lds, s);
+      [[fallthrough]];
     default:
       return occ ? occupancy_one<kIPX, kWL, kStatNone>(nb, wpg, lds)
                  : launch_one<kIPX, kWL, kStatNone>(pr, groups, wpg, lds, s);
@@ -2632,7 +2644,9 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   MCE_HIP(hipGetDevice(&dev));
   hipDeviceProp_t prop;
   MCE_HIP(hipGetDeviceProperties(&prop, dev));
-  const int stat = stat_of(cfg.task);
+  // the lane solver runs in its own kernel instance (IPX, no statistics)
+  const uint32_t ls_max = stat_of(cfg.task) == kStatNone && cfg.ipx ? std::min<uint32_t>(cfg.lane_max, kLsMax) : 0;
+  const int stat = ls_max ? kStatLane : stat_of(cfg.task);
   const uint32_t hist_lds = std::min<uint32_t>(hist_len, 128);
   const size_t hist_bytes = (8ull * hist_lds + 7) & ~7ull;
   const size_t lds_cu = 160 * 1024;
@@ -2653,7 +2667,6 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   }
   wpg = std::max<uint32_t>(1, std::min<uint32_t>(wpg, max_wpg));
   // the lane solver's regions (one per wave) follow the pool, 64-byte aligned
-  const uint32_t ls_max = std::min<uint32_t>(cfg.lane_max, kLsMax);
   const size_t ls_total = ls_max ? (size_t)kLsBytes * wpg + 64 : 0;
   const size_t fixed = hist_bytes + kPoolHeader + ls_total, share = lds_cu / groups_cu;
   size_t pool = cfg.lds_bytes ? (size_t)cfg.lds_bytes * wpg : share > fixed + kGranule ? share - fixed : kGranule;
@@ -2932,8 +2945,10 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
     return set_error(MCE_EINTERNAL, "%llu items did not fit their arena, %llu predicted", novf, hneed[1]);
   if (novf) {
     const uint64_t big = (hneed[0] + 255) & ~255ull;
+    // at most the first launch's workgroups: the lane solver's regions are
+    // sized for its waves
     const uint32_t g2 = (uint32_t)std::max<uint64_t>(
-        1, std::min<uint64_t>((novf + wpg - 1) / wpg, (freeb / 2) / ((uint64_t)wpg * big)));
+        1, std::min<uint64_t>({(novf + wpg - 1) / wpg, (freeb / 2) / ((uint64_t)wpg * big), (uint64_t)groups}));
     uint8_t *hbm2 = nullptr;
     if ((rc = dmalloc((void **)&hbm2, big * g2 * wpg, -1))) return rc;
     std::vector<uint64_t> idx(novf);
@@ -2957,7 +2972,8 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
     p2.overflow = nullptr;
     p2.overflow_cap = 0;
     MCE_HIP(hipMemsetAsync(ticket, 0, sizeof(uint32_t), s));
-    if ((rc = dispatch(cfg.ipx, false, kStatNone, false, nullptr, p2, g2, wpg, lds, s))) return rc;
+    if ((rc = dispatch(cfg.ipx, false, ls_max ? kStatLane : kStatNone, false, nullptr, p2, g2, wpg, lds, s)))
+      return rc;
     unsigned long long again = 0;
     MCE_HIP(hipMemcpyAsync(&again, &acc[kAccOverflow], sizeof(again), hipMemcpyDeviceToHost, s));
     MCE_HIP(hipStreamSynchronize(s));
