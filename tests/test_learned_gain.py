@@ -1,7 +1,7 @@
 """What the trained cost model gains over the method without it, on a graph
 it was not trained on. The committed Chung-Lu model (16 -> 64 -> 64 -> 1,
-trained on the GPU on seeds 1-8 at n = 2^18, beta 2.1, mean degree 24, n/2048
-planted cliques of 20..60) is evaluated on seed 9:
+trained on the GPU on seeds 1-8 at n = 2^18, 2^19 and 2^20, beta 2.1, mean
+degree 24, n/2048 planted cliques of 20..60) is evaluated on seed 9 at 2^18:
 
   - ranking: the Spearman correlation between its predictions and the
     measured BK tree size of every first-level subproblem beats that of the
