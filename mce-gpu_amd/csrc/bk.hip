@@ -1022,6 +1022,28 @@ __device__ __forceinline__ void fill_hash_terms(const Sub &s) {
 // every x). Lane k returns the keep mask of rows 64k..64k+63 (q <= 4096);
 // nkeep is their count. Rows with no bit are never candidates, so dropping
 // them (in order) changes nothing but the memory a subproblem needs.
+// Does outside vertex x have a later neighbour in S (ascending, p ids)? Only
+// such rows can cover a node of the subproblem.
+__device__ __forceinline__ bool row_live(const BkParams &pr, const uint32_t *S, uint32_t p, uint32_t x) {
+  for (uint64_t t = pr.split[x], e = pr.rp[x + 1]; t < e; ++t) {
+    const uint32_t y = pr.oc[t];
+    const uint32_t j = lower_bound_u32(S, p, y);
+    if (j < p && S[j] == y) return true;
+  }
+  return false;
+}
+
+// Number of live rows among the q vertices in ids, for any q.
+__device__ __forceinline__ uint32_t live_count(const BkParams &pr, const uint32_t *S, uint32_t p, const uint32_t *ids,
+                                               uint32_t q) {
+  uint32_t n = 0;
+  for (uint32_t b = 0; b < q; b += kWave) {
+    const uint32_t i = b + lane_id();
+    n += popc64(ballot(i < q && row_live(pr, S, p, ids[i])));
+  }
+  return n;
+}
+
 __device__ __forceinline__ uint64_t live_rows(const BkParams &pr, const uint32_t *S, uint32_t p, const uint32_t *ids,
                                               uint32_t q, uint32_t &nkeep) {
   const uint32_t L = lane_id();
@@ -1051,8 +1073,10 @@ __device__ __forceinline__ uint64_t all_rows(uint32_t q) { return full_word(q); 
 // Builds A over S and the kept outside rows (keep: mask from live_rows over
 // the q0 vertices in `ids`; s.q = their count), numbered in order; returns
 // this lane's word of the initial outside-row set (all kept rows).
+// filter: q0 may exceed 64 x 64 and the live rows are found chunk by chunk
+// instead of from keep (their count, s.q, was taken by live_count).
 __device__ __forceinline__ uint64_t build_bits(const BkParams &pr, Sub &s, const uint32_t *ids, uint32_t q0,
-                                               uint64_t keep) {
+                                               uint64_t keep, bool filter) {
   const uint32_t L = lane_id(), p = s.p, W = s.W, q = s.q;
   zero_words(s.A, (uint64_t)p * W);
   zero_words(s.XR, (uint64_t)q * W);
@@ -1074,7 +1098,7 @@ __device__ __forceinline__ uint64_t build_bits(const BkParams &pr, Sub &s, const
   }
   uint32_t base = 0;
   for (uint32_t b = 0, k = 0; b < q0; b += kWave, ++k) {
-    const uint64_t km = rdlane64(keep, k);
+    const uint64_t km = filter ? ballot(b + L < q0 && row_live(pr, s.S, p, ids[b + L])) : rdlane64(keep, k);
     if ((km >> L) & 1ull) {
       const uint32_t x = ids[b + L];
       uint64_t *row = s.XR + (uint64_t)(base + mbcnt(km)) * W;
@@ -2094,12 +2118,12 @@ __device__ __forceinline__ void finish_sub(const BkParams &pr, Sub &s, uint64_t 
 }
 
 template <bool kWL, int kStat>
-__device__ __forceinline__ void finish_bits(const BkParams &pr, Sub &s, const uint32_t *ids, uint32_t q0,
+__device__ __forceinline__ void finish_bits(const BkParams &pr, Sub &s, const uint32_t *ids, uint32_t q0, bool filter,
                                             uint64_t keep, unsigned long long *hist_lds, WaveAcc &wa,
                                             Clock<kStat> &clk) {
   const uint64_t nodes0 = wa.nodes;
   clk.to(kBdInduced);
-  const uint64_t E0 = build_bits(pr, s, ids, q0, keep);
+  const uint64_t E0 = build_bits(pr, s, ids, q0, keep, filter);
   solve_bits<kWL, kStat>(pr, s, full_word(s.p), 0, E0, hist_lds, wa, clk);
   clk.to(kBdOther);
   if (lane_id() == 0) {
@@ -2191,11 +2215,20 @@ __global__ __launch_bounds__(max_wpg_of(kWL) * kWave, waves_per_simd_of(kWL)) vo
         const uint32_t p = (uint32_t)(e0 - s0), q = (uint32_t)(s0 - b0);
         if (p == 0) continue;
         const uint32_t W = (p + 63) / 64;
-        const bool bits = kIPX && q <= kMaxXRows;
+        bool bits = kIPX && q <= kMaxXRows;
         // outside rows: all of them, or (many more than |P|, as at hubs) only
-        // those with a neighbour in P, in order
+        // those with a neighbour in P, in order; a hub with more than 4096
+        // earlier neighbours takes the bitmap path when its live rows fit
         uint32_t qb = q;
-        const uint64_t keep = !bits ? 0ull : q > 2 * p ? live_rows(pr, pr.oc + s0, p, pr.oc + b0, q, qb) : all_rows(q);
+        bool filter = false;
+        if (kIPX && !bits) {
+          qb = live_count(pr, pr.oc + s0, p, pr.oc + b0, q);
+          bits = filter = qb <= kMaxXRows;
+          if (!bits) qb = q;
+        }
+        const uint64_t keep = !bits || filter ? 0ull
+                              : q > 2 * p     ? live_rows(pr, pr.oc + s0, p, pr.oc + b0, q, qb)
+                                              : all_rows(q);
         const Region rg = bits ? pool_take_wait(pl, bits_want(p, qb, W, lv), bits_want_inner(p, qb, W, lv),
                                                 bits_fs(W, (qb + 63) / 64) * min(p, lv),
                                                 bits_want_inner(p, qb, W, lv))
@@ -2215,7 +2248,7 @@ __global__ __launch_bounds__(max_wpg_of(kWL) * kWave, waves_per_simd_of(kWL)) vo
         s.rmix = splitmix64(v);
         for (uint32_t i = L; i < p; i += kWave) s.S[i] = pr.oc[s0 + i];
         if (bits) {
-          finish_bits<kWL, kStat>(pr, s, pr.oc + b0, q, keep, hist_lds, wa, clk);
+          finish_bits<kWL, kStat>(pr, s, pr.oc + b0, q, filter, keep, hist_lds, wa, clk);
         } else {
           for (uint32_t i = L; i < q; i += kWave) s.XL[i] = pr.oc[b0 + i];
           finish_sub<kIPX, kWL, kStat>(pr, s, full_word(p), 0, hist_lds, wa, clk);
@@ -2265,7 +2298,7 @@ __global__ __launch_bounds__(max_wpg_of(kWL) * kWave, waves_per_simd_of(kWL)) vo
         s.rsize = 2;
         s.root = u;
         s.rmix = rmix;
-        if (bits) finish_bits<kWL, kStat>(pr, s, t.XL, q, keep, hist_lds, wa, clk);
+        if (bits) finish_bits<kWL, kStat>(pr, s, t.XL, q, false, keep, hist_lds, wa, clk);
         else finish_sub<kIPX, kWL, kStat>(pr, s, full_word(p), 0, hist_lds, wa, clk);
         pool_put(pl, rg);
       }

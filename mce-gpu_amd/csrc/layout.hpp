@@ -133,7 +133,10 @@ MCE_HD uint64_t bits_need(uint32_t p, uint32_t q) {
 //   layout. Four more frames of the bound widths cover the staged list's
 //   alignment slack.
 MCE_HD uint64_t item_need(uint32_t p, uint32_t q, bool arc, bool ipx) {
-  if (!arc) return ipx && q <= kMaxXRows ? bits_need(p, q) : hbm_need(p, q, ipx);
+  // a vertex with more outside rows takes the bitmap path when at most
+  // kMaxXRows of them are live, the list path otherwise
+  if (!arc) return !ipx ? hbm_need(p, q, ipx) : q <= kMaxXRows ? bits_need(p, q)
+                                                              : max64(hbm_need(p, q, ipx), bits_need(p, kMaxXRows));
   const uint64_t bits = ipx ? bits_need(p, q < kMaxXRows ? q : kMaxXRows) : 0ull;
   return max64(hbm_need(p, q, ipx), bits) + 4ull * bits_fs(words_of(p), words_of(q));
 }

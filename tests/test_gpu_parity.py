@@ -138,6 +138,44 @@ def test_lane_universe_thresholds(mce, oracle, lane):
             lds_arena_bytes=1024, worklist=2)
 
 
+def hub_graph(mce, live, dead, core=60, p_core=0.9, hub_core=40, seed=4):
+    """A hub with `live` + `dead` leaf neighbours and `hub_core` neighbours in
+    a dense core of `core` vertices. Leaves and the hub are peeled before the
+    core, so the hub's subproblem has P = its core neighbours and X = all its
+    leaves; a live leaf also touches one of those core neighbours, a dead
+    leaf only the hub."""
+    rng = np.random.default_rng(seed)
+    hub = 0
+    cs = np.arange(1, core + 1)
+    iu = np.triu_indices(core, 1)
+    keep = rng.random(iu[0].size) < p_core
+    src = [cs[iu[0][keep]], np.full(hub_core, hub)]
+    dst = [cs[iu[1][keep]], cs[:hub_core]]
+    leaves = np.arange(core + 1, core + 1 + live + dead)
+    src.append(np.full(leaves.size, hub))
+    dst.append(leaves)
+    src.append(leaves[:live])
+    dst.append(rng.choice(cs[:hub_core], live))
+    n = core + 1 + live + dead
+    return mce.from_edges(np.concatenate(src).astype(np.uint32), np.concatenate(dst).astype(np.uint32), n=n)
+
+
+@pytest.mark.parametrize("live,dead", [(3000, 2500), (4500, 200)])
+def test_hub_outside_rows_beyond_bitmap(mce, oracle, live, dead):
+    # a hub subproblem with more than 4096 outside rows: with at most 4096 of
+    # them live (a neighbour in P) it runs on the bitmap path over the live
+    # rows, otherwise on the list path
+    g = hub_graph(mce, live, dead)
+    ref = oracle.mce(g, level=1, ipx=True, sub_nodes=True)
+    with mce.DeviceGraph(g) as dg:
+        dg.order()
+        for wl in (1, 2):
+            print(f"case hub live={live} dead={dead} wl={wl}", flush=True)
+            res = dg.count(level=1, induced=2, worklist=wl, sub_nodes=True)
+            assert_same(res, ref, (live, dead, wl))
+            np.testing.assert_array_equal(res["sub_nodes"], ref["sub_nodes"])
+
+
 def test_hbm_arena_and_frame_spill(mce, oracle):
     # a tiny LDS arena forces the HBM arena and HBM stack frames
     g = rand_graph(mce, 220, 0.55, 7)

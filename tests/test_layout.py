@@ -25,7 +25,10 @@ static uint64_t worst_vertex(uint32_t p, uint32_t q, bool ipx, uint32_t lds, uin
       if (qk <= q) w = max64(w, bits_plan(p, qk, W, lds, st).hbm);
     return w;
   }
-  return list_hbm_need(p, q, W, ipx, lds, st);
+  uint64_t w = list_hbm_need(p, q, W, ipx, lds, st);
+  if (ipx)  // more outside rows, of which at most kMaxXRows live: the bitmap layout
+    for (uint32_t qk : {0u, kMaxXRows / 2, kMaxXRows}) w = max64(w, bits_plan(p, qk, W, lds, st).hbm);
+  return w;
 }
 
 static uint64_t worst_arc(uint32_t pb, uint32_t qb, bool ipx, uint32_t lds, uint32_t st) {
