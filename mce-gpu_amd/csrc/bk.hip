@@ -1022,25 +1022,45 @@ __device__ __forceinline__ void fill_hash_terms(const Sub &s) {
 // every x). Lane k returns the keep mask of rows 64k..64k+63 (q <= 4096);
 // nkeep is their count. Rows with no bit are never candidates, so dropping
 // them (in order) changes nothing but the memory a subproblem needs.
-// Does outside vertex x have a later neighbour in S (ascending, p ids)? Only
-// such rows can cover a node of the subproblem.
-__device__ __forceinline__ bool row_live(const BkParams &pr, const uint32_t *S, uint32_t p, uint32_t x) {
-  for (uint64_t t = pr.split[x], e = pr.rp[x + 1]; t < e; ++t) {
-    const uint32_t y = pr.oc[t];
-    const uint32_t j = lower_bound_u32(S, p, y);
-    if (j < p && S[j] == y) return true;
+// Rows kept from the chunk ids[b, b + 64) of a long outside list (more than
+// 64 x 64 rows): every row with two or more neighbours in S, and a row with
+// exactly one, S[j], only if it is the first such row for j (mark: one bit per
+// S index, word j >> 6 on lane j >> 6). A later copy of a row never decides
+// anything: it never wins a pivot tie against the earlier one (ties go to the
+// lower row), and it covers exactly the nodes the earlier one covers.
+__device__ __forceinline__ uint64_t keep_chunk(const BkParams &pr, const uint32_t *S, uint32_t p, const uint32_t *ids,
+                                               uint32_t q, uint32_t b, uint64_t &mark) {
+  const uint32_t L = lane_id(), i = b + L;
+  uint32_t hits = 0, j = 0;
+  if (i < q) {
+    const uint32_t x = ids[i];
+    for (uint64_t t = pr.split[x], e = pr.rp[x + 1]; t < e && hits < 2; ++t) {
+      const uint32_t y = pr.oc[t];
+      const uint32_t k = lower_bound_u32(S, p, y);
+      if (k < p && S[k] == y) {
+        if (hits == 0) j = k;
+        ++hits;
+      }
+    }
   }
-  return false;
+  uint64_t km = ballot(hits >= 2);
+  for (uint64_t pend = ballot(hits == 1); pend;) {
+    const uint32_t l = ctz64(pend), jl = rdlane(j, l);
+    pend &= ~ballot(hits == 1 && j == jl);
+    if (!((rdlane64(mark, jl >> 6) >> (jl & 63)) & 1ull)) {
+      km |= 1ull << l;
+      if (L == (jl >> 6)) mark |= 1ull << (jl & 63);
+    }
+  }
+  return km;
 }
 
-// Number of live rows among the q vertices in ids, for any q.
+// Number of rows keep_chunk keeps from the q vertices in ids, for any q.
 __device__ __forceinline__ uint32_t live_count(const BkParams &pr, const uint32_t *S, uint32_t p, const uint32_t *ids,
                                                uint32_t q) {
   uint32_t n = 0;
-  for (uint32_t b = 0; b < q; b += kWave) {
-    const uint32_t i = b + lane_id();
-    n += popc64(ballot(i < q && row_live(pr, S, p, ids[i])));
-  }
+  uint64_t mark = 0;
+  for (uint32_t b = 0; b < q; b += kWave) n += popc64(keep_chunk(pr, S, p, ids, q, b, mark));
   return n;
 }
 
@@ -1073,8 +1093,9 @@ __device__ __forceinline__ uint64_t all_rows(uint32_t q) { return full_word(q); 
 // Builds A over S and the kept outside rows (keep: mask from live_rows over
 // the q0 vertices in `ids`; s.q = their count), numbered in order; returns
 // this lane's word of the initial outside-row set (all kept rows).
-// filter: q0 may exceed 64 x 64 and the live rows are found chunk by chunk
-// instead of from keep (their count, s.q, was taken by live_count).
+// filter: q0 may exceed 64 x 64 and the kept rows are found chunk by chunk
+// (keep_chunk) instead of from keep; their count, s.q, was taken by
+// live_count.
 __device__ __forceinline__ uint64_t build_bits(const BkParams &pr, Sub &s, const uint32_t *ids, uint32_t q0,
                                                uint64_t keep, bool filter) {
   const uint32_t L = lane_id(), p = s.p, W = s.W, q = s.q;
@@ -1097,8 +1118,9 @@ __device__ __forceinline__ uint64_t build_bits(const BkParams &pr, Sub &s, const
     }
   }
   uint32_t base = 0;
+  uint64_t mark = 0;
   for (uint32_t b = 0, k = 0; b < q0; b += kWave, ++k) {
-    const uint64_t km = filter ? ballot(b + L < q0 && row_live(pr, s.S, p, ids[b + L])) : rdlane64(keep, k);
+    const uint64_t km = filter ? keep_chunk(pr, s.S, p, ids, q0, b, mark) : rdlane64(keep, k);
     if ((km >> L) & 1ull) {
       const uint32_t x = ids[b + L];
       uint64_t *row = s.XR + (uint64_t)(base + mbcnt(km)) * W;
@@ -2218,7 +2240,8 @@ __global__ __launch_bounds__(max_wpg_of(kWL) * kWave, waves_per_simd_of(kWL)) vo
         bool bits = kIPX && q <= kMaxXRows;
         // outside rows: all of them, or (many more than |P|, as at hubs) only
         // those with a neighbour in P, in order; a hub with more than 4096
-        // earlier neighbours takes the bitmap path when its live rows fit
+        // earlier neighbours takes the bitmap path when its live rows, less
+        // repeated one-neighbour rows, fit
         uint32_t qb = q;
         bool filter = false;
         if (kIPX && !bits) {

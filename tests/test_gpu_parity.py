@@ -138,12 +138,13 @@ def test_lane_universe_thresholds(mce, oracle, lane):
             lds_arena_bytes=1024, worklist=2)
 
 
-def hub_graph(mce, live, dead, core=60, p_core=0.9, hub_core=40, seed=4):
-    """A hub with `live` + `dead` leaf neighbours and `hub_core` neighbours in
-    a dense core of `core` vertices. Leaves and the hub are peeled before the
-    core, so the hub's subproblem has P = its core neighbours and X = all its
-    leaves; a live leaf also touches one of those core neighbours, a dead
-    leaf only the hub."""
+def hub_graph(mce, live, dead, live2=0, core=60, p_core=0.9, hub_core=40, seed=4):
+    """A hub with `live` + `dead` + `live2` leaf neighbours and `hub_core`
+    neighbours in a dense core of `core` vertices. Leaves and the hub are
+    peeled before the core, so the hub's subproblem has P = its core
+    neighbours and X = all its leaves; a live leaf also touches one of those
+    core neighbours (its outside row has one bit: many repeat), a `live2` leaf
+    two of them, a dead leaf only the hub."""
     rng = np.random.default_rng(seed)
     hub = 0
     cs = np.arange(1, core + 1)
@@ -156,21 +157,28 @@ def hub_graph(mce, live, dead, core=60, p_core=0.9, hub_core=40, seed=4):
     dst.append(leaves)
     src.append(leaves[:live])
     dst.append(rng.choice(cs[:hub_core], live))
-    n = core + 1 + live + dead
+    two = np.arange(core + 1 + live + dead, core + 1 + live + dead + live2)
+    src += [np.full(two.size, hub), two, two]
+    a = rng.integers(0, hub_core, live2)
+    b = (a + 1 + rng.integers(0, hub_core - 1, live2)) % hub_core
+    dst += [two, cs[a], cs[b]]
+    n = core + 1 + live + dead + live2
     return mce.from_edges(np.concatenate(src).astype(np.uint32), np.concatenate(dst).astype(np.uint32), n=n)
 
 
-@pytest.mark.parametrize("live,dead", [(3000, 2500), (4500, 200)])
-def test_hub_outside_rows_beyond_bitmap(mce, oracle, live, dead):
-    # a hub subproblem with more than 4096 outside rows: with at most 4096 of
-    # them live (a neighbour in P) it runs on the bitmap path over the live
-    # rows, otherwise on the list path
-    g = hub_graph(mce, live, dead)
+@pytest.mark.parametrize("live,dead,live2", [(3000, 2500, 0), (4500, 200, 0), (3000, 0, 3000), (100, 100, 4200)])
+def test_hub_outside_rows_beyond_bitmap(mce, oracle, live, dead, live2):
+    # a hub subproblem with more than 4096 outside rows: when at most 4096 of
+    # them are live (a neighbour in P) once repeated one-neighbour rows are
+    # dropped, it runs on the bitmap path over those rows (the first three
+    # cases: 3000 live; 40 distinct of 4500; 40 + 3000 of 6000), otherwise on
+    # the list path (4200 rows with two neighbours)
+    g = hub_graph(mce, live, dead, live2)
     ref = oracle.mce(g, level=1, ipx=True, sub_nodes=True)
     with mce.DeviceGraph(g) as dg:
         dg.order()
         for wl in (1, 2):
-            print(f"case hub live={live} dead={dead} wl={wl}", flush=True)
+            print(f"case hub live={live} dead={dead} live2={live2} wl={wl}", flush=True)
             res = dg.count(level=1, induced=2, worklist=wl, sub_nodes=True)
             assert_same(res, ref, (live, dead, wl))
             np.testing.assert_array_equal(res["sub_nodes"], ref["sub_nodes"])
