@@ -156,6 +156,7 @@ struct BkParams {
   uint32_t ls_max;
   uint32_t ls_off;
   uint64_t *ls_h;
+  uint64_t *cover;  // hub row filter: 64 words per wave
 };
 
 // Posts a progress value to pinned host memory (a system-scope vector store;
@@ -186,6 +187,7 @@ struct Sub {
   const uint32_t *oc;
   const uint64_t *rp;
   const uint64_t *split;
+  uint64_t *cover;  // hub row filter: the wave's 64 union words (device memory)
 };
 
 struct WaveAcc {
@@ -1024,17 +1026,24 @@ __device__ __forceinline__ void fill_hash_terms(const Sub &s) {
 // them (in order) changes nothing but the memory a subproblem needs.
 // Rows kept from the chunk ids[b, b + 64) of a long outside list (more than
 // 64 x 64 rows): every row with two or more neighbours in S, and a row with
-// exactly one, S[j], only if it is the first such row for j (mark: one bit per
-// S index, word j >> 6 on lane j >> 6). A later copy of a row never decides
-// anything: it never wins a pivot tie against the earlier one (ties go to the
-// lower row), and it covers exactly the nodes the earlier one covers.
+// exactly one, S[j], only if no earlier kept row contains j. Such a row never
+// decides anything: whenever it is live the earlier row is too (it holds the
+// row's only member), on every pivot score the earlier row scores at least as
+// much and wins a tie (lower row first), and it covers every node the row
+// covers. cover (64 words in device memory, the wave's own, zeroed by the
+// caller) holds the union of the rows kept so far; the chunk's singles are
+// decided in row order, each also against the chunk's earlier multi-member
+// rows, then those rows join the union.
 __device__ __forceinline__ uint64_t keep_chunk(const BkParams &pr, const uint32_t *S, uint32_t p, const uint32_t *ids,
-                                               uint32_t q, uint32_t b, uint64_t &mark) {
+                                               uint32_t q, uint32_t b, uint64_t *cover) {
   const uint32_t L = lane_id(), i = b + L;
-  uint32_t hits = 0, j = 0;
+  uint32_t hits = 0, j = 0, x = 0;
+  uint64_t t0 = 0, t1 = 0;
   if (i < q) {
-    const uint32_t x = ids[i];
-    for (uint64_t t = pr.split[x], e = pr.rp[x + 1]; t < e && hits < 2; ++t) {
+    x = ids[i];
+    t0 = pr.split[x];
+    t1 = pr.rp[x + 1];
+    for (uint64_t t = t0; t < t1 && hits < 2; ++t) {
       const uint32_t y = pr.oc[t];
       const uint32_t k = lower_bound_u32(S, p, y);
       if (k < p && S[k] == y) {
@@ -1043,24 +1052,48 @@ __device__ __forceinline__ uint64_t keep_chunk(const BkParams &pr, const uint32_
       }
     }
   }
-  uint64_t km = ballot(hits >= 2);
-  for (uint64_t pend = ballot(hits == 1); pend;) {
-    const uint32_t l = ctz64(pend), jl = rdlane(j, l);
-    pend &= ~ballot(hits == 1 && j == jl);
-    if (!((rdlane64(mark, jl >> 6) >> (jl & 63)) & 1ull)) {
+  const bool multi = hits >= 2;
+  uint64_t km = ballot(multi);
+  for (uint64_t pend = ballot(hits == 1); pend; pend &= pend - 1) {
+    const uint32_t l = ctz64(pend), jl = rdlane(j, l), yj = S[jl];
+    bool cov = (__hip_atomic_load(&cover[jl >> 6], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> (jl & 63)) & 1ull;
+    if (!cov) {
+      bool has = false;
+      if (L < l && multi) {
+        const uint64_t len = t1 - t0;
+        const uint32_t k = lower_bound_u32(pr.oc + t0, (uint32_t)len, yj);
+        has = k < len && pr.oc[t0 + k] == yj;
+      }
+      cov = ballot(has) != 0;
+    }
+    if (!cov) {
       km |= 1ull << l;
-      if (L == (jl >> 6)) mark |= 1ull << (jl & 63);
+      if (L == 0) atomicOr((unsigned long long *)&cover[jl >> 6], 1ull << (jl & 63));
+      __builtin_amdgcn_s_waitcnt(0);
     }
   }
+  if (multi)
+    for (uint64_t t = t0; t < t1; ++t) {
+      const uint32_t y = pr.oc[t];
+      const uint32_t k = lower_bound_u32(S, p, y);
+      if (k < p && S[k] == y) atomicOr((unsigned long long *)&cover[k >> 6], 1ull << (k & 63));
+    }
+  __builtin_amdgcn_s_waitcnt(0);
   return km;
+}
+
+// Clears the wave's cover words.
+__device__ __forceinline__ void clear_cover(uint64_t *cover) {
+  __hip_atomic_store(&cover[lane_id()], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __builtin_amdgcn_s_waitcnt(0);
 }
 
 // Number of rows keep_chunk keeps from the q vertices in ids, for any q.
 __device__ __forceinline__ uint32_t live_count(const BkParams &pr, const uint32_t *S, uint32_t p, const uint32_t *ids,
-                                               uint32_t q) {
+                                               uint32_t q, uint64_t *cover) {
   uint32_t n = 0;
-  uint64_t mark = 0;
-  for (uint32_t b = 0; b < q; b += kWave) n += popc64(keep_chunk(pr, S, p, ids, q, b, mark));
+  clear_cover(cover);
+  for (uint32_t b = 0; b < q; b += kWave) n += popc64(keep_chunk(pr, S, p, ids, q, b, cover));
   return n;
 }
 
@@ -1118,9 +1151,9 @@ __device__ __forceinline__ uint64_t build_bits(const BkParams &pr, Sub &s, const
     }
   }
   uint32_t base = 0;
-  uint64_t mark = 0;
+  if (filter) clear_cover(s.cover);
   for (uint32_t b = 0, k = 0; b < q0; b += kWave, ++k) {
-    const uint64_t km = filter ? keep_chunk(pr, s.S, p, ids, q0, b, mark) : rdlane64(keep, k);
+    const uint64_t km = filter ? keep_chunk(pr, s.S, p, ids, q0, b, s.cover) : rdlane64(keep, k);
     if ((km >> L) & 1ull) {
       const uint32_t x = ids[b + L];
       uint64_t *row = s.XR + (uint64_t)(base + mbcnt(km)) * W;
@@ -2190,6 +2223,7 @@ __global__ __launch_bounds__(max_wpg_of(kWL) * kWave, waves_per_simd_of(kWL)) vo
   s.oc = pr.oc;
   s.rp = pr.rp;
   s.split = pr.split;
+  s.cover = pr.cover + (uint64_t)gw * kWave;
   const uint32_t *items1 = (const uint32_t *)pr.items;
   const uint2 *items2 = (const uint2 *)pr.items;
   // phase 1: first-level subproblems by ticket
@@ -2245,7 +2279,7 @@ __global__ __launch_bounds__(max_wpg_of(kWL) * kWave, waves_per_simd_of(kWL)) vo
         uint32_t qb = q;
         bool filter = false;
         if (kIPX && !bits) {
-          qb = live_count(pr, pr.oc + s0, p, pr.oc + b0, q);
+          qb = live_count(pr, pr.oc + s0, p, pr.oc + b0, q, s.cover);
           bits = filter = qb <= kMaxXRows;
           if (!bits) qb = q;
         }
@@ -2829,8 +2863,9 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   unsigned long long *wave_val = nullptr, *wave_bd = nullptr;
   uint32_t *wave_cu = nullptr;
   if (arena && (rc = dmalloc((void **)&hbm, arena * waves, kWsArena))) return rc;
-  uint64_t *ls_h = nullptr;
+  uint64_t *ls_h = nullptr, *cover = nullptr;
   if (ls_max && (rc = dmalloc((void **)&ls_h, (uint64_t)kLsGlobalBytes * waves, kWsLane))) return rc;
+  if (cfg.ipx && (rc = dmalloc((void **)&cover, 8ull * kWave * waves, kWsCover))) return rc;
   if (hneed[1] && (rc = dmalloc((void **)&overflow, sizeof(uint64_t) * hneed[1], kWsOverflow))) return rc;
   if (wave_stats) {
     if ((rc = dmalloc((void **)&wave_val, sizeof(unsigned long long) * waves, -1))) return rc;
@@ -2856,6 +2891,7 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   pr.ls_max = ls_max;
   pr.ls_off = (uint32_t)ls_off;
   pr.ls_h = ls_h;
+  pr.cover = cover;
   pr.watchdog_ticks = cfg.watchdog_us ? (uint64_t)cfg.watchdog_us * 100 : kWatchdogTicks;
   pr.hbm = hbm;
   pr.hbm_bytes = arena;

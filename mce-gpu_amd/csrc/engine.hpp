@@ -57,6 +57,7 @@ enum WsSlot : int {
   kWsIdle,
   kWsMbox,
   kWsLane,      // lane solver: hash terms, headers and frames of every wave
+  kWsCover,     // hub row filter: 64 words per wave
   kWsSlots
 };
 struct Workspace {

@@ -138,13 +138,14 @@ def test_lane_universe_thresholds(mce, oracle, lane):
             lds_arena_bytes=1024, worklist=2)
 
 
-def hub_graph(mce, live, dead, live2=0, core=60, p_core=0.9, hub_core=40, seed=4):
+def hub_graph(mce, live, dead, live2=0, two_first=False, core=60, p_core=0.9, hub_core=40, seed=4):
     """A hub with `live` + `dead` + `live2` leaf neighbours and `hub_core`
     neighbours in a dense core of `core` vertices. Leaves and the hub are
     peeled before the core, so the hub's subproblem has P = its core
     neighbours and X = all its leaves; a live leaf also touches one of those
     core neighbours (its outside row has one bit: many repeat), a `live2` leaf
-    two of them, a dead leaf only the hub."""
+    two of them, a dead leaf only the hub. Outside rows are in vertex order:
+    the `live2` leaves come last, or first with two_first."""
     rng = np.random.default_rng(seed)
     hub = 0
     cs = np.arange(1, core + 1)
@@ -152,12 +153,14 @@ def hub_graph(mce, live, dead, live2=0, core=60, p_core=0.9, hub_core=40, seed=4
     keep = rng.random(iu[0].size) < p_core
     src = [cs[iu[0][keep]], np.full(hub_core, hub)]
     dst = [cs[iu[1][keep]], cs[:hub_core]]
-    leaves = np.arange(core + 1, core + 1 + live + dead)
+    first = core + 1 + (live2 if two_first else 0)
+    leaves = np.arange(first, first + live + dead)
     src.append(np.full(leaves.size, hub))
     dst.append(leaves)
     src.append(leaves[:live])
     dst.append(rng.choice(cs[:hub_core], live))
-    two = np.arange(core + 1 + live + dead, core + 1 + live + dead + live2)
+    t0 = core + 1 if two_first else core + 1 + live + dead
+    two = np.arange(t0, t0 + live2)
     src += [np.full(two.size, hub), two, two]
     a = rng.integers(0, hub_core, live2)
     b = (a + 1 + rng.integers(0, hub_core - 1, live2)) % hub_core
@@ -166,19 +169,22 @@ def hub_graph(mce, live, dead, live2=0, core=60, p_core=0.9, hub_core=40, seed=4
     return mce.from_edges(np.concatenate(src).astype(np.uint32), np.concatenate(dst).astype(np.uint32), n=n)
 
 
-@pytest.mark.parametrize("live,dead,live2", [(3000, 2500, 0), (4500, 200, 0), (3000, 0, 3000), (100, 100, 4200)])
-def test_hub_outside_rows_beyond_bitmap(mce, oracle, live, dead, live2):
+@pytest.mark.parametrize("live,dead,live2,two_first", [(3000, 2500, 0, False), (4500, 200, 0, False),
+                                                      (3000, 0, 3000, False), (100, 100, 4200, False),
+                                                      (500, 0, 4080, False), (500, 0, 4080, True)])
+def test_hub_outside_rows_beyond_bitmap(mce, oracle, live, dead, live2, two_first):
     # a hub subproblem with more than 4096 outside rows: when at most 4096 of
-    # them are live (a neighbour in P) once repeated one-neighbour rows are
-    # dropped, it runs on the bitmap path over those rows (the first three
-    # cases: 3000 live; 40 distinct of 4500; 40 + 3000 of 6000), otherwise on
-    # the list path (4200 rows with two neighbours)
-    g = hub_graph(mce, live, dead, live2)
+    # them are kept (a neighbour in P; a one-neighbour row only if no earlier
+    # kept row holds that neighbour) it runs on the bitmap path over them
+    # (3000 live; 40 distinct of 4500; 40 + 3000 of 6000; 4080 two-neighbour
+    # rows first, covering the 500 one-neighbour rows after them), otherwise
+    # on the list path (4238 rows; 40 first one-neighbour rows + 4080)
+    g = hub_graph(mce, live, dead, live2, two_first)
     ref = oracle.mce(g, level=1, ipx=True, sub_nodes=True)
     with mce.DeviceGraph(g) as dg:
         dg.order()
         for wl in (1, 2):
-            print(f"case hub live={live} dead={dead} live2={live2} wl={wl}", flush=True)
+            print(f"case hub live={live} dead={dead} live2={live2} two_first={two_first} wl={wl}", flush=True)
             res = dg.count(level=1, induced=2, worklist=wl, sub_nodes=True)
             assert_same(res, ref, (live, dead, wl))
             np.testing.assert_array_equal(res["sub_nodes"], ref["sub_nodes"])
