@@ -2801,6 +2801,11 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   ticket = (uint32_t *)(acc + kAccN + 2);
   unsigned long long *need = acc + kAccN;
   MCE_HIP(hipMemsetAsync(acc, 0, sizeof(unsigned long long) * (kAccN + 2), s));
+  // hub row filter: 64 words per wave, for every wave a launch can hold (four
+  // per SIMD), taken before the arenas size themselves to what is left
+  uint64_t *cover = nullptr;
+  const uint32_t max_waves = (uint32_t)prop.multiProcessorCount * 16;
+  if (cfg.ipx && (rc = dmalloc((void **)&cover, 8ull * kWave * max_waves, kWsCover))) return rc;
   // per-wave HBM arena cap: items that need more are deferred to a second
   // pass with arenas sized for the largest. From a scratch region, one
   // workgroup's arenas must fit what is left of it, and there is no second
@@ -2863,9 +2868,9 @@ int bk_run(const DevGraph &g, const OrderInfo &info, const void *d_items, uint64
   unsigned long long *wave_val = nullptr, *wave_bd = nullptr;
   uint32_t *wave_cu = nullptr;
   if (arena && (rc = dmalloc((void **)&hbm, arena * waves, kWsArena))) return rc;
-  uint64_t *ls_h = nullptr, *cover = nullptr;
+  uint64_t *ls_h = nullptr;
   if (ls_max && (rc = dmalloc((void **)&ls_h, (uint64_t)kLsGlobalBytes * waves, kWsLane))) return rc;
-  if (cfg.ipx && (rc = dmalloc((void **)&cover, 8ull * kWave * waves, kWsCover))) return rc;
+  if (cover && waves > max_waves) return set_error(MCE_EINTERNAL, "%u waves exceed the hub filter's %u", waves, max_waves);
   if (hneed[1] && (rc = dmalloc((void **)&overflow, sizeof(uint64_t) * hneed[1], kWsOverflow))) return rc;
   if (wave_stats) {
     if ((rc = dmalloc((void **)&wave_val, sizeof(unsigned long long) * waves, -1))) return rc;
